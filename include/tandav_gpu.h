@@ -1,0 +1,149 @@
+/*
+ * tandav_gpu.h -- C ABI of the MI355X gravity engine for Tandav.
+ *
+ * One context owns one GPU's copy of the particle columns, the top nodes,
+ * the octree and the Ewald table, and runs the per-step force path on the
+ * device: PH keys -> radix sort -> reorder -> top nodes -> octree build ->
+ * tree walk (+ Ewald walk) -> kick/drift.  Every entry point returns 0 on
+ * success or a TDV_ERR_* code; tdv_last_error() gives the message.  No C++
+ * exception crosses this boundary.  Calls on one context are not thread
+ * safe (the reference calls its team-collective functions from an OpenMP
+ * team; the drop-in layer funnels them through one thread).
+ *
+ * Each declaration names the reference function (jdonnert/Tandav) whose
+ * work it performs.  "host arrays" are caller-owned host memory; all other
+ * state lives in device memory inside the context.
+ */
+#ifndef TANDAV_GPU_H
+#define TANDAV_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+	TDV_OK = 0,
+	TDV_ERR_ARG = 1,          /* bad argument / call order               */
+	TDV_ERR_HIP = 2,          /* HIP runtime error                        */
+	TDV_ERR_NOMEM = 3,        /* device allocation failed                 */
+	TDV_ERR_DUP_KEYS = 4,     /* two particles share a 128-bit PH key     */
+	TDV_ERR_KEY_DEPTH = 5,    /* particles closer than the key resolution */
+	TDV_ERR_RANGE = 6,        /* a particle range left the arrays         */
+	TDV_ERR_TIES = 7          /* a run of equal 64-bit key prefixes too long */
+};
+
+typedef struct tdv_context tdv_context;
+
+/* Config / parameter-file knobs of the force path, at run time. */
+typedef struct {
+	int periodic;          /* PERIODIC (Config:7)                              */
+	double boxsize;        /* Sim.Boxsize[0] (periodic runs)                   */
+	double gravity;        /* Const.Gravity (constants.c:3-8)                  */
+	double softening;      /* Param.Grav_Softening[1], Plummer equivalent      */
+	double theta_bh;       /* TREE_OPEN_PARAM_BH (Config:47)                   */
+	double alpha_rel;      /* TREE_OPEN_PARAM_REL (Config:48)                  */
+	int vector_size;       /* VECTOR_SIZE (Config:54), must be 64              */
+	int compat;            /* 1: small top nodes read particles
+	                          [Target, Target+Npart) like
+	                          interact_with_topnode_particles
+	                          (Gravity/tree_accel.c:183-203); 0: their own
+	                          particles                                        */
+	int device;            /* HIP device ordinal                               */
+} tdv_params;
+
+/* Setup_Gravity_Tree (Gravity/tree_build.c:124-142) and, for periodic runs,
+ * Gravity_Periodic_Init (Gravity/periodic.c:180-233): softening lengths,
+ * device buffers and the FP64 Ewald table computed on the GPU. */
+int tdv_create(const tdv_params *p, tdv_context **out);
+
+/* Gravity_Tree_Free (Gravity/tree_build.c:144-156) plus everything else. */
+void tdv_destroy(tdv_context *ctx);
+
+const char *tdv_last_error(const tdv_context *ctx);
+
+/* Copy the particle columns P.Pos/Vel/Mass/Acc/Last_Acc_Mag/Time_Bin/ID
+ * (particles.h:17-37) from host arrays to the device.  Any pointer but
+ * pos and mass may be NULL (zero / all-active / identity). */
+int tdv_set_particles(tdv_context *ctx, size_t n, const double *pos[3],
+		const double *vel[3], const double *mass, const double *acc[3],
+		const double *last_acc_mag, const int *time_bin, const uint32_t *id);
+
+/* Copy device columns back to host arrays (NULL skips a column).  key is
+ * 2*n uint64 words (lo, hi) holding P.Key as the reference leaves it:
+ * reversed keys after a domain decomposition (peano.c:84-97). */
+int tdv_get_particles(tdv_context *ctx, double *pos[3], double *vel[3],
+		double *mass, double *acc[3], double *grav_acc[3], double *cost,
+		double *last_acc_mag, int *tree_parent, uint64_t *key, uint32_t *id,
+		int *time_bin);
+
+/* Setup_Domain_Decomposition (domain.c:151-189): computational domain,
+ * then Sort_Particles_By_Peano_Key (peano.c:20-48). */
+int tdv_setup_domain(tdv_context *ctx);
+
+/* Domain_Decomposition (domain.c:67-145) for one rank, NTarget = 1:
+ * domain, PH keys, key sort, reorder of every column, level-2 top nodes in
+ * descending key order, reversed keys (Reverse_Peano_Keys, peano.c:84-97),
+ * active list (Make_Active_Particle_List, timestep.c:280-299). */
+int tdv_domain_decomposition(tdv_context *ctx);
+
+/* Gravity_Tree_Build (Gravity/tree_build.c:47-122) from the sorted keys. */
+int tdv_tree_build(tdv_context *ctx);
+
+/* Gravity_Tree_Acceleration (Gravity/tree_accel.c:32-91) followed, when
+ * periodic, by Gravity_Tree_Periodic (Gravity/tree_periodic.c:23-72), with
+ * the BH (use_bh = 1) or relative criterion, FP64, one lane per active
+ * particle. */
+int tdv_tree_acceleration(tdv_context *ctx, int use_bh);
+
+/* Compute_Acceleration (accel.c:15-71): |a| into Last_Acc_Mag, build if a
+ * decomposition happened, BH walk + relative walk on the prepare step,
+ * relative walk otherwise. */
+int tdv_compute_acceleration(tdv_context *ctx, int prepare_step);
+
+/* Element-wise batched forms of the pure functions (host arrays). */
+/* Peano_Key / Reversed_Peano_Key (peano.c:122-276); keys_out = 2n words. */
+int tdv_peano_keys(tdv_context *ctx, size_t n, const double *x,
+		const double *y, const double *z, const double origin[3], double size,
+		int reversed, uint64_t *keys_out);
+
+/* Qsort_Index(perm, keys, n, 16, cmp_peanoKeys) (sort.c:90-116): the
+ * permutation that sorts unique 128-bit keys ascending; keys = 2n words. */
+int tdv_sort_keys(tdv_context *ctx, size_t n, const uint64_t *keys,
+		uint32_t *perm_out);
+
+/* Ewald_Correction (Gravity/periodic.c:35-113) for n separations
+ * dr[3n] -> f[3n]. */
+int tdv_ewald_correction(tdv_context *ctx, size_t n, const double *dr,
+		double *f);
+
+/* read/write_ewald_correction_table (Gravity/periodic.c:303-342): the
+ * unnormalised 4 x 65^3 table (Fx, Fy, Fz, Fp) in box units. */
+int tdv_get_ewald_table(tdv_context *ctx, double *table4);
+int tdv_set_ewald_table(tdv_context *ctx, const double *table4);
+
+/* Domain (domain.h:58-63) after the last decomposition. */
+int tdv_get_domain(tdv_context *ctx, double *size, double origin[3],
+		double center[3]);
+
+/* D[].TNode (domain.h:40-52), 64-byte records in the reference layout. */
+int tdv_get_top_nodes(tdv_context *ctx, void *out, int max, int *ntop);
+
+/* Tree and NNodes (Gravity/tree.h:23-39), 96-byte Tree_Node records. */
+int tdv_get_tree(tdv_context *ctx, void *out, size_t max, size_t *nnodes);
+
+/* Wall times (ms, HIP events on the context's stream) of the stages of the
+ * last call: 0 keys, 1 sort, 2 reorder, 3 top nodes, 4 build, 5 walk,
+ * 6 Ewald walk, 7 kick/drift. */
+int tdv_stage_times(tdv_context *ctx, float ms[8]);
+
+/* Number of particles currently on the device. */
+size_t tdv_num_particles(const tdv_context *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,763 @@
+/*
+ * C ABI of the engine (include/tandav_gpu.h): context, device buffers,
+ * host <-> device copies and the step drivers that chain the kernels in the
+ * reference's order (domain.c:67-145, accel.c:15-71).
+ */
+#include "tdv_internal.h"
+#include "tdv_device.h"
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#define TDV_EBIT_MASK (TDV_EBIT_DUP | TDV_EBIT_DEPTH | TDV_EBIT_RANGE | TDV_EBIT_TIES)
+
+enum { EV_KEYS0, EV_KEYS1, EV_SORT1, EV_REORDER1, EV_TOP1, EV_BUILD0,
+	EV_BUILD1, EV_WALK0, EV_WALK1, EV_EWALD1, EV_KICK0, EV_KICK1, EV_N };
+
+int tdv_fail(tdv_context *c, int code, const std::string &msg)
+{
+	if (c)
+		c->err = msg;
+	return code;
+}
+
+int tdv_check(tdv_context *c, hipError_t e, const char *what)
+{
+	if (e == hipSuccess)
+		return 0;
+	(void)hipGetLastError();
+	return tdv_fail(c, e == hipErrorOutOfMemory ? TDV_ERR_NOMEM : TDV_ERR_HIP,
+			std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int tdv_check_err_flags(tdv_context *c)
+{
+	unsigned e = 0;
+	TDV_HIP(c, hipMemcpyAsync(&e, c->d_err, sizeof(e), hipMemcpyDeviceToHost,
+			c->stream));
+	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	if (!e)
+		return 0;
+	TDV_HIP(c, hipMemsetAsync(c->d_err, 0, sizeof(unsigned), c->stream));
+	if (e & TDV_EBIT_DUP)
+		return tdv_fail(c, TDV_ERR_DUP_KEYS,
+				"two particles share a 128-bit Peano-Hilbert key");
+	if (e & TDV_EBIT_DEPTH)
+		return tdv_fail(c, TDV_ERR_KEY_DEPTH,
+				"particles closer than the Peano-Hilbert key resolution");
+	if (e & TDV_EBIT_TIES)
+		return tdv_fail(c, TDV_ERR_TIES, "run of equal 64-bit key prefixes");
+	return tdv_fail(c, TDV_ERR_RANGE,
+			"a small top node's particle range leaves the particle arrays");
+}
+
+template <typename T>
+static int dalloc(tdv_context *c, T **p, size_t count)
+{
+	*p = nullptr;
+	if (count == 0)
+		count = 1;
+	return tdv_check(c, hipMalloc((void **)p, count * sizeof(T)), "hipMalloc");
+}
+
+template <typename T>
+static void dfree(T *&p)
+{
+	if (p)
+		(void)hipFree((void *)p);
+	p = nullptr;
+}
+
+static void free_particles(tdv_context *c)
+{
+	for (int d = 0; d < 3; d++) {
+		dfree(c->pos[d]);
+		dfree(c->vel[d]);
+		dfree(c->acc[d]);
+		dfree(c->gacc[d]);
+	}
+	dfree(c->mass);
+	dfree(c->last_acc);
+	dfree(c->cost);
+	dfree(c->tmp_f64);
+	dfree(c->key);
+	dfree(c->key_tmp);
+	dfree(c->tree_parent);
+	dfree(c->time_bin);
+	dfree(c->id);
+	dfree(c->pm);
+	dfree(c->active);
+	for (int b = 0; b < 2; b++) {
+		dfree(c->sk[b]);
+		dfree(c->sv[b]);
+	}
+	dfree(c->hist);
+	dfree(c->scan_part);
+	dfree(c->perm);
+	dfree(c->minpack);
+	dfree(c->cnt);
+	dfree(c->off);
+	dfree(c->lohi);
+	c->cap = 0;
+}
+
+static int alloc_particles(tdv_context *c, size_t n)
+{
+	free_particles(c);
+	for (int d = 0; d < 3; d++) {
+		TDV_TRY(dalloc(c, &c->pos[d], n));
+		TDV_TRY(dalloc(c, &c->vel[d], n));
+		TDV_TRY(dalloc(c, &c->acc[d], n));
+		TDV_TRY(dalloc(c, &c->gacc[d], n));
+	}
+	TDV_TRY(dalloc(c, &c->mass, n));
+	TDV_TRY(dalloc(c, &c->last_acc, n));
+	TDV_TRY(dalloc(c, &c->cost, n));
+	TDV_TRY(dalloc(c, &c->tmp_f64, n));
+	TDV_TRY(dalloc(c, &c->key, 2 * n));
+	TDV_TRY(dalloc(c, &c->key_tmp, 2 * n));
+	TDV_TRY(dalloc(c, &c->tree_parent, n));
+	TDV_TRY(dalloc(c, &c->time_bin, n));
+	TDV_TRY(dalloc(c, &c->id, n));
+	TDV_TRY(dalloc(c, &c->pm, n));
+	TDV_TRY(dalloc(c, &c->active, n));
+	for (int b = 0; b < 2; b++) {
+		TDV_TRY(dalloc(c, &c->sk[b], n));
+		TDV_TRY(dalloc(c, &c->sv[b], n));
+	}
+	const size_t nb = (n + 4095) / 4096;
+	c->hist_cap = nb * 256 + 1;
+	TDV_TRY(dalloc(c, &c->hist, c->hist_cap));
+	const size_t sp = (c->hist_cap > n + 1 ? c->hist_cap : n + 1) / 1024 + 2;
+	TDV_TRY(dalloc(c, &c->scan_part, sp));
+	TDV_TRY(dalloc(c, &c->perm, n));
+	TDV_TRY(dalloc(c, &c->minpack, n));
+	TDV_TRY(dalloc(c, &c->cnt, n));
+	TDV_TRY(dalloc(c, &c->off, n + 1));
+	TDV_TRY(dalloc(c, &c->lohi, n));
+	c->cap = n;
+	return 0;
+}
+
+int tdv_ensure_nodes(tdv_context *c, size_t nodes)
+{
+	if (nodes <= c->node_cap && c->node_a)
+		return 0;
+	dfree(c->node_a);
+	dfree(c->node_b);
+	dfree(c->node_c);
+	dfree(c->node_up);
+	/* Max_Nodes grows by TREE_ENLARGEMENT_FACTOR (tree_build.c:6,162) */
+	size_t cap = (size_t)(1.2 * (double)nodes) + 1024;
+	TDV_TRY(dalloc(c, &c->node_a, cap));
+	TDV_TRY(dalloc(c, &c->node_b, cap));
+	TDV_TRY(dalloc(c, &c->node_c, cap));
+	TDV_TRY(dalloc(c, &c->node_up, cap));
+	c->node_cap = cap;
+	return 0;
+}
+
+static int alloc_top(tdv_context *c)
+{
+	TopNodes &t = c->top;
+	TDV_TRY(dalloc(c, &t.first, TDV_MAX_TOP));
+	TDV_TRY(dalloc(c, &t.npart, TDV_MAX_TOP));
+	TDV_TRY(dalloc(c, &t.level, TDV_MAX_TOP));
+	TDV_TRY(dalloc(c, &t.target, TDV_MAX_TOP));
+	TDV_TRY(dalloc(c, &t.term, TDV_MAX_TOP));
+	TDV_TRY(dalloc(c, &t.key, TDV_MAX_TOP));
+	TDV_TRY(dalloc(c, &t.pos, 3 * TDV_MAX_TOP));
+	TDV_TRY(dalloc(c, &t.mass, TDV_MAX_TOP));
+	TDV_TRY(dalloc(c, &t.com, 3 * TDV_MAX_TOP));
+	TDV_TRY(dalloc(c, &t.cell2top, 64));
+	TDV_TRY(dalloc(c, &t.ntop, 1));
+	TDV_TRY(dalloc(c, &t.nnodes, 1));
+	TDV_HIP(c, hipMemset(t.ntop, 0, sizeof(int)));
+	return 0;
+}
+
+static void free_top(tdv_context *c)
+{
+	TopNodes &t = c->top;
+	dfree(t.first);
+	dfree(t.npart);
+	dfree(t.level);
+	dfree(t.target);
+	dfree(t.term);
+	dfree(t.key);
+	dfree(t.pos);
+	dfree(t.mass);
+	dfree(t.com);
+	dfree(t.cell2top);
+	dfree(t.ntop);
+	dfree(t.nnodes);
+}
+
+static int compute_ewald(tdv_context *c)
+{
+	double *raw = nullptr;
+	TDV_TRY(dalloc(c, &raw, 4 * (size_t)TDV_NE3));
+	int r = tdv_ewald_table_device(c, raw);
+	if (!r) {
+		std::vector<double> h(4 * (size_t)TDV_NE3);
+		r = tdv_check(c, hipMemcpyAsync(h.data(), raw, h.size() * 8,
+				hipMemcpyDeviceToHost, c->stream), "ewald copy");
+		if (!r)
+			r = tdv_check(c, hipStreamSynchronize(c->stream), "ewald sync");
+		if (!r)
+			r = tdv_set_ewald_table(c, h.data());
+	}
+	dfree(raw);
+	return r;
+}
+
+extern "C" {
+
+int tdv_create(const tdv_params *p, tdv_context **out)
+{
+	if (!p || !out)
+		return TDV_ERR_ARG;
+	*out = nullptr;
+	if (p->vector_size != 64)
+		return TDV_ERR_ARG;
+	if (p->periodic && !(p->boxsize > 0))
+		return TDV_ERR_ARG;
+	tdv_context *c = new (std::nothrow) tdv_context();
+	if (!c)
+		return TDV_ERR_NOMEM;
+	c->par = *p;
+	/* Setup_Gravity_Tree, tree_build.c:134-139: K1 length, negative sign */
+	c->Epsilon = -41.0 / 32.0 * p->softening;
+	c->Epsilon2 = c->Epsilon * c->Epsilon;
+	c->Epsilon3 = c->Epsilon * c->Epsilon * c->Epsilon;
+	int r = tdv_check(c, hipSetDevice(p->device), "hipSetDevice");
+	if (!r)
+		r = tdv_check(c, hipStreamCreateWithFlags(&c->stream,
+				hipStreamNonBlocking), "hipStreamCreate");
+	for (int e = 0; !r && e < EV_N; e++)
+		r = tdv_check(c, hipEventCreate(&c->ev[e]), "hipEventCreate");
+	if (!r)
+		r = dalloc(c, &c->d_err, 1);
+	if (!r)
+		r = tdv_check(c, hipMemset(c->d_err, 0, sizeof(unsigned)), "memset");
+	if (!r)
+		r = alloc_top(c);
+	if (!r && p->periodic)
+		r = compute_ewald(c);       /* Gravity_Periodic_Init */
+	if (r) {
+		fprintf(stderr, "tdv_create: %s\n", c->err.c_str());
+		tdv_destroy(c);
+		return r;
+	}
+	*out = c;
+	return 0;
+}
+
+void tdv_destroy(tdv_context *c)
+{
+	if (!c)
+		return;
+	if (c->stream)
+		(void)hipStreamSynchronize(c->stream);
+	free_particles(c);
+	free_top(c);
+	dfree(c->node_a);
+	dfree(c->node_b);
+	dfree(c->node_c);
+	dfree(c->node_up);
+	dfree(c->ewald);
+	dfree(c->d_err);
+	for (int e = 0; e < EV_N; e++)
+		if (c->ev[e])
+			(void)hipEventDestroy(c->ev[e]);
+	if (c->stream)
+		(void)hipStreamDestroy(c->stream);
+	delete c;
+}
+
+const char *tdv_last_error(const tdv_context *c)
+{
+	return c ? c->err.c_str() : "no context";
+}
+
+size_t tdv_num_particles(const tdv_context *c)
+{
+	return c ? c->n : 0;
+}
+
+int tdv_set_particles(tdv_context *c, size_t n, const double *pos[3],
+		const double *vel[3], const double *mass, const double *acc[3],
+		const double *last_acc_mag, const int *time_bin, const uint32_t *id)
+{
+	if (!c || !pos || !mass || n == 0 || n > 0x7FFFFFFFu)
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_set_particles: bad arguments");
+	if (n > c->cap)
+		TDV_TRY(alloc_particles(c, n));
+	c->n = n;
+	const size_t b = n * sizeof(double);
+	for (int d = 0; d < 3; d++) {
+		TDV_HIP(c, hipMemcpyAsync(c->pos[d], pos[d], b, hipMemcpyHostToDevice,
+				c->stream));
+		if (vel && vel[d])
+			TDV_HIP(c, hipMemcpyAsync(c->vel[d], vel[d], b,
+					hipMemcpyHostToDevice, c->stream));
+		else
+			TDV_HIP(c, hipMemsetAsync(c->vel[d], 0, b, c->stream));
+		if (acc && acc[d])
+			TDV_HIP(c, hipMemcpyAsync(c->acc[d], acc[d], b,
+					hipMemcpyHostToDevice, c->stream));
+		else
+			TDV_HIP(c, hipMemsetAsync(c->acc[d], 0, b, c->stream));
+		TDV_HIP(c, hipMemsetAsync(c->gacc[d], 0, b, c->stream));
+	}
+	TDV_HIP(c, hipMemcpyAsync(c->mass, mass, b, hipMemcpyHostToDevice,
+			c->stream));
+	if (last_acc_mag)
+		TDV_HIP(c, hipMemcpyAsync(c->last_acc, last_acc_mag, b,
+				hipMemcpyHostToDevice, c->stream));
+	else
+		TDV_HIP(c, hipMemsetAsync(c->last_acc, 0, b, c->stream));
+	TDV_HIP(c, hipMemsetAsync(c->cost, 0, b, c->stream));
+	TDV_HIP(c, hipMemsetAsync(c->tree_parent, 0, n * sizeof(int), c->stream));
+	if (time_bin)
+		TDV_HIP(c, hipMemcpyAsync(c->time_bin, time_bin, n * sizeof(int),
+				hipMemcpyHostToDevice, c->stream));
+	else
+		TDV_HIP(c, hipMemsetAsync(c->time_bin, 0, n * sizeof(int), c->stream));
+	if (id) {
+		TDV_HIP(c, hipMemcpyAsync(c->id, id, n * sizeof(uint32_t),
+				hipMemcpyHostToDevice, c->stream));
+	} else {
+		std::vector<uint32_t> ids(n);
+		for (size_t i = 0; i < n; i++)
+			ids[i] = (uint32_t)i;
+		TDV_HIP(c, hipMemcpyAsync(c->id, ids.data(), n * sizeof(uint32_t),
+				hipMemcpyHostToDevice, c->stream));
+		TDV_HIP(c, hipStreamSynchronize(c->stream));
+	}
+	/* every particle active (Time_Bin <= Max_Active_Bin on a sync point) */
+	std::vector<int> act(n);
+	for (size_t i = 0; i < n; i++)
+		act[i] = (int)i;
+	TDV_HIP(c, hipMemcpyAsync(c->active, act.data(), n * sizeof(int),
+			hipMemcpyHostToDevice, c->stream));
+	c->nactive = (int)n;
+	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	c->have_domain = 0;
+	c->have_tree = 0;
+	tdv_launch_pack_pm(c);
+	return tdv_check(c, hipStreamSynchronize(c->stream), "set_particles");
+}
+
+int tdv_get_particles(tdv_context *c, double *pos[3], double *vel[3],
+		double *mass, double *acc[3], double *grav_acc[3], double *cost,
+		double *last_acc_mag, int *tree_parent, uint64_t *key, uint32_t *id,
+		int *time_bin)
+{
+	if (!c)
+		return TDV_ERR_ARG;
+	const size_t n = c->n, b = n * sizeof(double);
+	auto cp = [&](void *h, const void *d, size_t bytes) {
+		return tdv_check(c, hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost,
+				c->stream), "get_particles");
+	};
+	for (int d = 0; d < 3; d++) {
+		if (pos && pos[d])
+			TDV_TRY(cp(pos[d], c->pos[d], b));
+		if (vel && vel[d])
+			TDV_TRY(cp(vel[d], c->vel[d], b));
+		if (acc && acc[d])
+			TDV_TRY(cp(acc[d], c->acc[d], b));
+		if (grav_acc && grav_acc[d])
+			TDV_TRY(cp(grav_acc[d], c->gacc[d], b));
+	}
+	if (mass)
+		TDV_TRY(cp(mass, c->mass, b));
+	if (cost)
+		TDV_TRY(cp(cost, c->cost, b));
+	if (last_acc_mag)
+		TDV_TRY(cp(last_acc_mag, c->last_acc, b));
+	if (tree_parent)
+		TDV_TRY(cp(tree_parent, c->tree_parent, n * sizeof(int)));
+	if (id)
+		TDV_TRY(cp(id, c->id, n * sizeof(uint32_t)));
+	if (time_bin)
+		TDV_TRY(cp(time_bin, c->time_bin, n * sizeof(int)));
+	if (key) {
+		if (c->have_domain) {       /* P.Key holds reversed keys */
+			tdv_launch_reverse_keys(c, n, c->key, c->key_tmp);
+			TDV_TRY(cp(key, c->key_tmp, 2 * n * sizeof(uint64_t)));
+		} else {
+			memset(key, 0, 2 * n * sizeof(uint64_t));
+		}
+	}
+	return tdv_check(c, hipStreamSynchronize(c->stream), "get_particles");
+}
+
+/* domain + keys + sort + reorder of every column (the part shared by
+ * Setup_Domain_Decomposition and Domain_Decomposition) */
+static int domain_keys_sort(tdv_context *c)
+{
+	const size_t n = c->n;
+	if (!n)
+		return tdv_fail(c, TDV_ERR_ARG, "no particles");
+	if (c->par.periodic) {                 /* domain.c:732-747 */
+		c->dom_size = c->par.boxsize;
+		for (int d = 0; d < 3; d++) {
+			c->dom_center[d] = 0.5 * c->dom_size;
+			c->dom_origin[d] = 0;
+		}
+	} else {
+		TDV_TRY(tdv_domain_nonperiodic(c));
+	}
+	(void)hipEventRecord(c->ev[EV_KEYS0], c->stream);
+	tdv_launch_keys(c, n, c->pos[0], c->pos[1], c->pos[2], c->dom_origin,
+			c->dom_size, c->key_tmp, 0);
+	(void)hipEventRecord(c->ev[EV_KEYS1], c->stream);
+	TDV_TRY(tdv_sort_device(c, n, c->key_tmp, c->perm));
+	(void)hipEventRecord(c->ev[EV_SORT1], c->stream);
+	/* reorder_collisionless_particles (peano.c:50-82) as gathers */
+	tdv_launch_gather_key(c, n, c->perm, c->key_tmp, c->key);
+	double **cols[] = { &c->pos[0], &c->pos[1], &c->pos[2], &c->vel[0],
+		&c->vel[1], &c->vel[2], &c->acc[0], &c->acc[1], &c->acc[2],
+		&c->gacc[0], &c->gacc[1], &c->gacc[2], &c->mass, &c->last_acc,
+		&c->cost };
+	for (double **col : cols) {
+		tdv_launch_gather_f64(c, n, c->perm, *col, c->tmp_f64);
+		double *t = *col;
+		*col = c->tmp_f64;
+		c->tmp_f64 = t;
+	}
+	int *icols[] = { c->tree_parent, c->time_bin,
+		reinterpret_cast<int *>(c->id) };
+	for (int *col : icols) {
+		tdv_launch_gather_i32(c, n, c->perm, col, c->lohi);
+		TDV_HIP(c, hipMemcpyAsync(col, c->lohi, n * sizeof(int),
+				hipMemcpyDeviceToDevice, c->stream));
+	}
+	tdv_launch_pack_pm(c);
+	(void)hipEventRecord(c->ev[EV_REORDER1], c->stream);
+	TDV_HIP(c, hipGetLastError());
+	TDV_TRY(tdv_check_err_flags(c));
+	c->have_domain = 1;
+	c->have_tree = 0;
+	return 0;
+}
+
+int tdv_setup_domain(tdv_context *c)
+{
+	if (!c)
+		return TDV_ERR_ARG;
+	return domain_keys_sort(c);
+}
+
+int tdv_domain_decomposition(tdv_context *c)
+{
+	if (!c)
+		return TDV_ERR_ARG;
+	TDV_TRY(domain_keys_sort(c));
+	TDV_TRY(tdv_build_top_nodes(c));
+	(void)hipEventRecord(c->ev[EV_TOP1], c->stream);
+	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	float ms;
+	(void)hipEventElapsedTime(&ms, c->ev[EV_KEYS0], c->ev[EV_KEYS1]);
+	c->stage_ms[0] = ms;
+	(void)hipEventElapsedTime(&ms, c->ev[EV_KEYS1], c->ev[EV_SORT1]);
+	c->stage_ms[1] = ms;
+	(void)hipEventElapsedTime(&ms, c->ev[EV_SORT1], c->ev[EV_REORDER1]);
+	c->stage_ms[2] = ms;
+	(void)hipEventElapsedTime(&ms, c->ev[EV_REORDER1], c->ev[EV_TOP1]);
+	c->stage_ms[3] = ms;
+	return 0;
+}
+
+int tdv_tree_build(tdv_context *c)
+{
+	if (!c || !c->have_domain)
+		return tdv_fail(c, TDV_ERR_ARG, "tree build before a decomposition");
+	(void)hipEventRecord(c->ev[EV_BUILD0], c->stream);
+	TDV_TRY(tdv_build_tree(c));
+	(void)hipEventRecord(c->ev[EV_BUILD1], c->stream);
+	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	(void)hipEventElapsedTime(&c->stage_ms[4], c->ev[EV_BUILD0], c->ev[EV_BUILD1]);
+	return tdv_check_err_flags(c);
+}
+
+int tdv_tree_acceleration(tdv_context *c, int use_bh)
+{
+	if (!c || !c->have_tree)
+		return tdv_fail(c, TDV_ERR_ARG, "walk before a tree build");
+	(void)hipEventRecord(c->ev[EV_WALK0], c->stream);
+	TDV_TRY(tdv_walk64(c, use_bh));
+	(void)hipEventRecord(c->ev[EV_WALK1], c->stream);
+	if (c->par.periodic)
+		TDV_TRY(tdv_walk64_ewald(c, use_bh));
+	(void)hipEventRecord(c->ev[EV_EWALD1], c->stream);
+	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	(void)hipEventElapsedTime(&c->stage_ms[5], c->ev[EV_WALK0], c->ev[EV_WALK1]);
+	(void)hipEventElapsedTime(&c->stage_ms[6], c->ev[EV_WALK1], c->ev[EV_EWALD1]);
+	return tdv_check_err_flags(c);
+}
+
+int tdv_compute_acceleration(tdv_context *c, int prepare_step)
+{
+	if (!c || !c->have_domain)
+		return tdv_fail(c, TDV_ERR_ARG, "acceleration before a decomposition");
+	tdv_launch_safe_last_accel(c);
+	if (!c->have_tree)
+		TDV_TRY(tdv_tree_build(c));
+	if (prepare_step) {
+		TDV_TRY(tdv_tree_acceleration(c, 1));
+		tdv_launch_safe_last_accel(c);
+	}
+	return tdv_tree_acceleration(c, 0);
+}
+
+int tdv_peano_keys(tdv_context *c, size_t n, const double *x, const double *y,
+		const double *z, const double origin[3], double size, int reversed,
+		uint64_t *keys_out)
+{
+	if (!c || !x || !y || !z || !keys_out)
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_peano_keys: bad arguments");
+	if (!n)
+		return 0;
+	double *d[3] = { nullptr, nullptr, nullptr };
+	uint64_t *k = nullptr;
+	int r = 0;
+	const double *h[3] = { x, y, z };
+	for (int q = 0; q < 3 && !r; q++) {
+		r = dalloc(c, &d[q], n);
+		if (!r)
+			r = tdv_check(c, hipMemcpyAsync(d[q], h[q], n * 8,
+					hipMemcpyHostToDevice, c->stream), "copy");
+	}
+	if (!r)
+		r = dalloc(c, &k, 2 * n);
+	if (!r) {
+		tdv_launch_keys(c, n, d[0], d[1], d[2], origin, size, k, reversed);
+		r = tdv_check(c, hipMemcpyAsync(keys_out, k, 16 * n,
+				hipMemcpyDeviceToHost, c->stream), "copy");
+	}
+	if (!r)
+		r = tdv_check(c, hipStreamSynchronize(c->stream), "keys");
+	for (int q = 0; q < 3; q++)
+		dfree(d[q]);
+	dfree(k);
+	return r;
+}
+
+int tdv_sort_keys(tdv_context *c, size_t n, const uint64_t *keys,
+		uint32_t *perm_out)
+{
+	if (!c || !keys || !perm_out)
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_sort_keys: bad arguments");
+	if (!n)
+		return 0;
+	if (n > c->cap) {        /* grows the scratch; drops particle state */
+		TDV_TRY(alloc_particles(c, n));
+		c->n = 0;
+		c->have_domain = c->have_tree = 0;
+	}
+	TDV_HIP(c, hipMemcpyAsync(c->key_tmp, keys, 16 * n, hipMemcpyHostToDevice,
+			c->stream));
+	TDV_TRY(tdv_sort_device(c, n, c->key_tmp, c->perm));
+	TDV_HIP(c, hipMemcpyAsync(perm_out, c->perm, 4 * n, hipMemcpyDeviceToHost,
+			c->stream));
+	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	return tdv_check_err_flags(c);
+}
+
+int tdv_ewald_correction(tdv_context *c, size_t n, const double *dr,
+		double *f)
+{
+	if (!c || !dr || !f || !c->have_ewald)
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_ewald_correction: no table");
+	if (!n)
+		return 0;
+	double *ddr = nullptr, *df = nullptr;
+	int r = dalloc(c, &ddr, 3 * n);
+	if (!r)
+		r = dalloc(c, &df, 3 * n);
+	if (!r)
+		r = tdv_check(c, hipMemcpyAsync(ddr, dr, 24 * n, hipMemcpyHostToDevice,
+				c->stream), "copy");
+	if (!r) {
+		tdv_launch_ewald_correction(c, n, ddr, df);
+		r = tdv_check(c, hipMemcpyAsync(f, df, 24 * n, hipMemcpyDeviceToHost,
+				c->stream), "copy");
+	}
+	if (!r)
+		r = tdv_check(c, hipStreamSynchronize(c->stream), "ewald");
+	dfree(ddr);
+	dfree(df);
+	return r;
+}
+
+int tdv_get_ewald_table(tdv_context *c, double *t4)
+{
+	if (!c || !t4 || !c->have_ewald)
+		return tdv_fail(c, TDV_ERR_ARG, "no Ewald table");
+	std::vector<double> h(4 * (size_t)TDV_NE3);
+	TDV_HIP(c, hipMemcpyAsync(h.data(), c->ewald, h.size() * 8,
+			hipMemcpyDeviceToHost, c->stream));
+	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	const double box = c->par.boxsize;
+	for (size_t i = 0; i < (size_t)TDV_NE3; i++) {
+		/* undo the 1/Box^2, 1/Box normalisation */
+		for (int q = 0; q < 3; q++)
+			t4[q * TDV_NE3 + i] = h[q * TDV_NE3 + i] * (box * box);
+		t4[3 * TDV_NE3 + i] = h[3 * TDV_NE3 + i] * box;
+	}
+	return 0;
+}
+
+int tdv_set_ewald_table(tdv_context *c, const double *t4)
+{
+	if (!c || !t4 || !c->par.periodic)
+		return tdv_fail(c, TDV_ERR_ARG, "Ewald table on a non-periodic run");
+	/* Gravity_Periodic_Init, Gravity/periodic.c:215-228 */
+	std::vector<double> h(4 * (size_t)TDV_NE3);
+	const double box = c->par.boxsize;
+	for (size_t i = 0; i < (size_t)TDV_NE3; i++) {
+		for (int q = 0; q < 3; q++)
+			h[q * TDV_NE3 + i] = t4[q * TDV_NE3 + i] / (box * box);
+		h[3 * TDV_NE3 + i] = t4[3 * TDV_NE3 + i] / box;
+	}
+	if (!c->ewald)
+		TDV_TRY(dalloc(c, &c->ewald, h.size()));
+	TDV_HIP(c, hipMemcpyAsync(c->ewald, h.data(), h.size() * 8,
+			hipMemcpyHostToDevice, c->stream));
+	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	c->have_ewald = 1;
+	return 0;
+}
+
+int tdv_get_domain(tdv_context *c, double *size, double origin[3],
+		double center[3])
+{
+	if (!c || !c->have_domain)
+		return tdv_fail(c, TDV_ERR_ARG, "no domain yet");
+	if (size)
+		*size = c->dom_size;
+	for (int d = 0; d < 3; d++) {
+		if (origin)
+			origin[d] = c->dom_origin[d];
+		if (center)
+			center[d] = c->dom_center[d];
+	}
+	return 0;
+}
+
+/* Top_Tree_Node layout, domain.h:40-52 */
+struct TopRecord {
+	uint64_t Key;
+	int Target, Level, First_Part, Npart;
+	float Pos[3], Mass, CoM[3], Dp[3];
+};
+
+int tdv_get_top_nodes(tdv_context *c, void *out, int max, int *ntop)
+{
+	if (!c || !out || !ntop || !c->have_domain)
+		return tdv_fail(c, TDV_ERR_ARG, "no top nodes yet");
+	const int n = c->ntop;
+	*ntop = n;
+	if (n > max)
+		return tdv_fail(c, TDV_ERR_ARG, "top node buffer too small");
+	std::vector<int> first(n), npart(n), level(n), target(n);
+	std::vector<uint64_t> key(n);
+	std::vector<float> pos(3 * n), mass(n), com(3 * n);
+	auto cp = [&](void *h, const void *d, size_t b) {
+		return b ? tdv_check(c, hipMemcpyAsync(h, d, b, hipMemcpyDeviceToHost,
+				c->stream), "top copy") : 0;
+	};
+	TDV_TRY(cp(first.data(), c->top.first, n * 4));
+	TDV_TRY(cp(npart.data(), c->top.npart, n * 4));
+	TDV_TRY(cp(level.data(), c->top.level, n * 4));
+	TDV_TRY(cp(target.data(), c->top.target, n * 4));
+	TDV_TRY(cp(key.data(), c->top.key, n * 8));
+	TDV_TRY(cp(pos.data(), c->top.pos, n * 12));
+	TDV_TRY(cp(mass.data(), c->top.mass, n * 4));
+	TDV_TRY(cp(com.data(), c->top.com, n * 12));
+	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	TopRecord *rec = static_cast<TopRecord *>(out);
+	for (int t = 0; t < n; t++) {
+		TopRecord r;
+		memset(&r, 0, sizeof(r));
+		r.Key = key[t];
+		r.Target = c->have_tree ? target[t] : -1;
+		r.Level = level[t];
+		r.First_Part = first[t];
+		r.Npart = npart[t];
+		for (int d = 0; d < 3; d++) {
+			r.Pos[d] = pos[3 * t + d];
+			r.CoM[d] = c->have_tree ? com[3 * t + d] : 0.f;
+		}
+		r.Mass = c->have_tree ? mass[t] : 0.f;
+		rec[t] = r;
+	}
+	return 0;
+}
+
+/* Tree_Node layout, Gravity/tree.h:23-32 (Float = double) */
+struct NodeRecord {
+	int DNext;
+	uint32_t Bitfield;
+	int DUp, Npart;
+	double Pos[3], Mass, CoM[3], Dp[3];
+};
+
+int tdv_get_tree(tdv_context *c, void *out, size_t max, size_t *nnodes)
+{
+	if (!c || !nnodes || !c->have_tree)
+		return tdv_fail(c, TDV_ERR_ARG, "no tree yet");
+	const size_t n = c->nnodes;
+	*nnodes = n;
+	if (!out)
+		return 0;
+	if (n > max)
+		return tdv_fail(c, TDV_ERR_ARG, "tree buffer too small");
+	if (!n)
+		return 0;
+	std::vector<double4> a(n), b(n);
+	std::vector<int4> cc(n);
+	std::vector<int> up(n);
+	TDV_HIP(c, hipMemcpyAsync(a.data(), c->node_a, n * 32,
+			hipMemcpyDeviceToHost, c->stream));
+	TDV_HIP(c, hipMemcpyAsync(b.data(), c->node_b, n * 32,
+			hipMemcpyDeviceToHost, c->stream));
+	TDV_HIP(c, hipMemcpyAsync(cc.data(), c->node_c, n * 16,
+			hipMemcpyDeviceToHost, c->stream));
+	TDV_HIP(c, hipMemcpyAsync(up.data(), c->node_up, n * 4,
+			hipMemcpyDeviceToHost, c->stream));
+	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	NodeRecord *rec = static_cast<NodeRecord *>(out);
+	for (size_t j = 0; j < n; j++) {
+		NodeRecord r;
+		memset(&r, 0, sizeof(r));
+		r.DNext = cc[j].x;
+		r.Bitfield = (uint32_t)cc[j].z;
+		r.DUp = up[j];
+		r.Npart = cc[j].y;
+		r.Pos[0] = b[j].x;
+		r.Pos[1] = b[j].y;
+		r.Pos[2] = b[j].z;
+		r.Mass = a[j].w;
+		r.CoM[0] = a[j].x;
+		r.CoM[1] = a[j].y;
+		r.CoM[2] = a[j].z;
+		rec[j] = r;
+	}
+	return 0;
+}
+
+int tdv_stage_times(tdv_context *c, float ms[8])
+{
+	if (!c || !ms)
+		return TDV_ERR_ARG;
+	memcpy(ms, c->stage_ms, sizeof(c->stage_ms));
+	return 0;
+}
+
+} /* extern "C" */

@@ -1,0 +1,458 @@
+/*
+ * Top nodes and the octree, built bottom-up from the sorted keys in
+ * coalesced particle-parallel passes.
+ *
+ * Top nodes (domain.c:67-145, 220-313 with NTarget = 1): the non-empty
+ * level-2 PH cells in descending key order, each with a float centre.
+ *
+ * Tree (Gravity/tree_build.c:47-572): the reference inserts particles one
+ * by one; this build derives the same node set and the same depth-first
+ * array from per-particle quantities (first divergence levels of adjacent
+ * keys, the reference's collapse events and the cells they collapse):
+ *   k_tree_marks   events and collapsed cells -> min collapse level per
+ *                  particle (atomicMin over <= 64 particles per event)
+ *   k_tree_counts  nodes owned by each particle: its chain of cells from
+ *                  dprev(i) down to its leaf or collapsed bundle
+ *   scan           node offsets in (first particle, level) order, which is
+ *                  the reference's depth-first order
+ *   k_tree_targets subtree sizes and Target offsets of the top nodes
+ *   k_tree_nodes   one lane per chain: a forward scan over the chain's
+ *                  largest cell gives every chain node its particle count,
+ *                  skip link, and the running FP64 sums of m x and m in PH
+ *                  order (bit-identical to the reference's incremental
+ *                  sums), plus the parent links of the chains it contains
+ *   k_tree_centres centres by the reference's sign rule, top-down along
+ *                  the parent links from the float top-node centre
+ */
+#include "tdv_internal.h"
+#include "tdv_device.h"
+
+#define NOCOLLAPSE 0xFFFFFFFFu
+
+/* ------------------------------------------------------------------------ */
+/* top nodes                                                                 */
+/* ------------------------------------------------------------------------ */
+
+__device__ __forceinline__ size_t lower_bound_cell(const uint64_t *key,
+		size_t n, unsigned cell)
+{
+	size_t lo = 0, hi = n;
+	while (lo < hi) {
+		const size_t mid = (lo + hi) >> 1;
+		if ((unsigned)(key[2 * mid + 1] >> 58) < cell)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	return lo;
+}
+
+__global__ void __launch_bounds__(64) k_top_nodes(size_t n,
+		const uint64_t *__restrict__ key, const double *__restrict__ x,
+		const double *__restrict__ y, const double *__restrict__ z, double o0,
+		double o1, double o2, double dom_size, TopNodes t)
+{
+	__shared__ int nonempty[64];
+	const unsigned cell = threadIdx.x;
+	const size_t f = lower_bound_cell(key, n, cell);
+	const size_t e = lower_bound_cell(key, n, cell + 1);
+	const int np = (int)(e - f);
+	nonempty[cell] = np > 0;
+	__syncthreads();
+	int rank = 0, total = 0;
+	for (int q = 0; q < 64; q++) {
+		total += nonempty[q];
+		if (q > (int)cell)
+			rank += nonempty[q];
+	}
+	if (cell == 0)
+		*t.ntop = total;
+	if (np == 0) {
+		t.cell2top[cell] = -1;
+		return;
+	}
+	t.cell2top[cell] = rank;
+	t.first[rank] = (int)f;
+	t.npart[rank] = np;
+	t.level[rank] = TDV_TOP_LEVEL;
+	t.key[rank] = ((uint64_t)cell << 58) | (~0ull >> 6);
+	const double size = dom_size / (double)(1ull << TDV_TOP_LEVEL);
+	const double p[3] = { x[f] - o0, y[f] - o1, z[f] - o2 };
+	const double o[3] = { o0, o1, o2 };
+	for (int d = 0; d < 3; d++)
+		t.pos[3 * rank + d] = (float)((floor(p[d] / size) + 0.5) * size + o[d]);
+	t.target[rank] = -1;
+}
+
+int tdv_build_top_nodes(tdv_context *c)
+{
+	hipLaunchKernelGGL(k_top_nodes, dim3(1), dim3(64), 0, c->stream, c->n,
+			c->key, c->pos[0], c->pos[1], c->pos[2], c->dom_origin[0],
+			c->dom_origin[1], c->dom_origin[2], c->dom_size, c->top);
+	TDV_HIP(c, hipGetLastError());
+	TDV_HIP(c, hipMemcpyAsync(&c->ntop, c->top.ntop, sizeof(int),
+			hipMemcpyDeviceToHost, c->stream));
+	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* tree                                                                      */
+/* ------------------------------------------------------------------------ */
+
+struct TopRange {
+	int t, first, last;     /* top node, its particle range */
+	bool tree;              /* top node holds > VECTOR_SIZE particles */
+};
+
+__device__ __forceinline__ TopRange top_of(const uint64_t *key, size_t i,
+		const TopNodes &T, int vs)
+{
+	const int cell = (int)(key[2 * i + 1] >> 58);
+	TopRange r;
+	r.t = T.cell2top[cell];
+	r.first = T.first[r.t];
+	r.last = r.first + T.npart[r.t] - 1;
+	r.tree = T.npart[r.t] > vs;
+	return r;
+}
+
+/* dprev of particle i: D(i-1, i), the top level for the first particle */
+__device__ __forceinline__ int dprev_of(const uint64_t *key, int i, int first)
+{
+	if (i == first)
+		return TDV_TOP_LEVEL;
+	return div_level(load_key(key, i - 1), load_key(key, i));
+}
+
+/* particles in the cell of level lvl that ends at i-1 (bounded scan:
+ * returns vs+1 when the cell holds more than vs particles) */
+__device__ __forceinline__ int cell_count_back(const uint64_t *key, int i,
+		int first, int lvl, int vs)
+{
+	int s = i - 1;
+	Key2 ks = load_key(key, s);
+	while (s > first && i - s <= vs) {
+		const Key2 kp = load_key(key, s - 1);
+		if (div_level(kp, ks) <= lvl)
+			break;
+		ks = kp;
+		s--;
+	}
+	return (i - s <= vs) ? i - s : vs + 1;
+}
+
+__global__ void __launch_bounds__(256) k_tree_marks(size_t n,
+		const uint64_t *__restrict__ key, TopNodes T, int vs,
+		uint32_t *__restrict__ minpack, unsigned *err)
+{
+	const size_t gi = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (gi >= n)
+		return;
+	const int i = (int)gi;
+	const TopRange r = top_of(key, gi, T, vs);
+	if (!r.tree || i == r.first)
+		return;
+	const int dp = dprev_of(key, i, r.first);
+	if (dp >= TDV_NT) {          /* below key resolution: not supported */
+		atomicOr(err, TDV_EBIT_DEPTH);
+		return;
+	}
+	const int dpp = dprev_of(key, i - 1, r.first);
+	const bool newb = (i >= r.first + 2) && dp < dpp;
+	if (!(newb || i == r.last))
+		return;
+	int lk = -1, cnt = 0;
+	const int c1 = cell_count_back(key, i, r.first, dp, vs);
+	if (c1 <= vs) {
+		lk = dp;
+		cnt = c1;
+	} else if (newb) {
+		const int c2 = cell_count_back(key, i, r.first, dpp - 1, vs);
+		if (c2 <= vs) {
+			lk = dpp - 1;
+			cnt = c2;
+		}
+	}
+	if (lk < 0)
+		return;
+	const uint32_t packed = ((uint32_t)lk << 24) | (uint32_t)cnt;
+	for (int q = i - cnt; q < i; q++)
+		atomicMin(&minpack[q], packed);
+}
+
+__global__ void __launch_bounds__(256) k_tree_counts(size_t n,
+		const uint64_t *__restrict__ key, TopNodes T, int vs,
+		const uint32_t *__restrict__ minpack, uint32_t *__restrict__ cnt,
+		int *__restrict__ lohi)
+{
+	const size_t gi = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (gi >= n)
+		return;
+	const int i = (int)gi;
+	const TopRange r = top_of(key, gi, T, vs);
+	if (!r.tree) {
+		cnt[i] = 0;
+		lohi[i] = 0;
+		return;
+	}
+	const int lo = dprev_of(key, i, r.first);
+	const int dn = (i == r.last) ? -1 : div_level(load_key(key, i),
+			load_key(key, i + 1));
+	const int lf = lo > dn ? lo : dn;
+	const uint32_t mp = minpack[i];
+	const int ml = (mp == NOCOLLAPSE) ? 99 : (int)(mp >> 24);
+	const int hi = lf < ml ? lf : ml;
+	const int c = hi >= lo ? hi - lo + 1 : 0;
+	cnt[i] = (uint32_t)c;
+	lohi[i] = lo | (hi << 8) | ((ml <= lf) ? (1 << 16) : 0);
+}
+
+__global__ void __launch_bounds__(64) k_tree_targets(TopNodes T, int vs,
+		const uint32_t *__restrict__ off)
+{
+	if (threadIdx.x != 0)
+		return;
+	const int ntop = *T.ntop;
+	int run = 0;
+	for (int t = 0; t < ntop; t++) {
+		int sz = 0;
+		if (T.npart[t] > vs)
+			sz = (int)(off[T.first[t] + T.npart[t]] - off[T.first[t]]) + 1;
+		T.target[t] = run;           /* reserve_tree_memory(nNeeded) */
+		T.term[t] = run + sz - 1;
+		run += sz;
+	}
+	*T.nnodes = run;
+}
+
+__device__ __forceinline__ int node_index(const TopNodes &T, int t, int first,
+		const uint32_t *off, int p, int last)
+{
+	if (p > last)
+		return T.term[t];
+	return T.target[t] + (int)(off[p] - off[first]);
+}
+
+__global__ void __launch_bounds__(256) k_tree_nodes(size_t n,
+		const uint64_t *__restrict__ key, const double *__restrict__ x,
+		const double *__restrict__ y, const double *__restrict__ z,
+		const double *__restrict__ m, TopNodes T, int vs,
+		const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ off,
+		const int *__restrict__ lohi, double4 *__restrict__ na,
+		int4 *__restrict__ nc, int *__restrict__ up,
+		int *__restrict__ tree_parent)
+{
+	const size_t gi = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (gi >= n)
+		return;
+	const int i = (int)gi;
+	if (cnt[i] == 0)
+		return;
+	const TopRange r = top_of(key, gi, T, vs);
+	const int lo = lohi[i] & 0xFF, hi = (lohi[i] >> 8) & 0xFF;
+	const int base = node_index(T, r.t, r.first, off, i, r.last);
+	const Key2 ki = load_key(key, i);
+
+	double sx = 0, sy = 0, sz = 0, sm = 0;
+	int cur = hi;
+	int k = i;
+	int runmin = 99;
+	Key2 kprev = ki;
+	for (;;) {
+		const double mk = m[k];
+		const double tx = x[k] * mk, ty = y[k] * mk, tz = z[k] * mk;
+		sx = sx + tx;
+		sy = sy + ty;
+		sz = sz + tz;
+		sm = sm + mk;
+		const int kn = k + 1;
+		int dn = -1;
+		Key2 kk = kprev;
+		if (kn <= r.last) {
+			kk = load_key(key, kn);
+			dn = div_level(kprev, kk);
+		}
+		if (dn <= cur) {                     /* cells of levels >= dn end at k */
+			const int lowest = dn > lo ? dn : lo;
+			const int next = node_index(T, r.t, r.first, off, kn, r.last);
+			for (int lv = lowest; lv <= cur; lv++) {
+				const int j = base + (lv - lo);
+				const bool leaf = lv == hi;
+				int bits = lv | ((int)key_triplet(ki, lv) << 6)
+						| (1 << TDV_BIT_LOCAL);
+				const bool root = j == T.target[r.t];
+				if (root)
+					bits |= 1 << TDV_BIT_TOP;
+				const int dnext = leaf ? (-i - 1) : (next - j);
+				na[j] = make_double4(sx / sm, sy / sm, sz / sm, sm);
+				nc[j] = make_int4(dnext, kn - i, bits, i);
+				/* the DUp of a chain head comes from its parent's scan */
+				if (root)
+					up[j] = r.t;
+				else if (lv > lo)
+					up[j] = 1;
+				if (leaf)
+					for (int q = i; q < kn; q++)
+						tree_parent[q] = j;
+			}
+			cur = lowest - 1;
+			if (cur < lo)
+				break;
+		}
+		/* kn heads a chain whose parent (level dn-1) is in this chain when
+		 * no particle between i and kn closes that level */
+		if (kn <= r.last && runmin >= dn && cnt[kn] > 0) {
+			const int jk = node_index(T, r.t, r.first, off, kn, r.last);
+			const int jp = base + (dn - 1 - lo);
+			up[jk] = jk - jp;
+		}
+		if (dn < runmin)
+			runmin = dn;
+		kprev = kk;
+		k = kn;
+	}
+}
+
+__global__ void k_tree_small_top(const double *__restrict__ x,
+		const double *__restrict__ y, const double *__restrict__ z,
+		const double *__restrict__ m, TopNodes T, int vs,
+		const double4 *__restrict__ na, int *__restrict__ tree_parent)
+{
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= *T.ntop)
+		return;
+	const int f = T.first[t], np = T.npart[t];
+	double cx, cy, cz, mm;
+	if (np > vs) {
+		const double4 a = na[T.target[t]];
+		cx = a.x;
+		cy = a.y;
+		cz = a.z;
+		mm = a.w;
+	} else {          /* finalise_subtree on a discarded subtree */
+		double sx = 0, sy = 0, sz = 0;
+		mm = 0;
+		for (int q = f; q < f + np; q++) {
+			sx = sx + x[q] * m[q];
+			sy = sy + y[q] * m[q];
+			sz = sz + z[q] * m[q];
+			mm = mm + m[q];
+			tree_parent[q] = -t - 1;
+		}
+		cx = sx / mm;
+		cy = sy / mm;
+		cz = sz / mm;
+	}
+	T.mass[t] = (float)mm;
+	T.com[3 * t] = (float)cx;
+	T.com[3 * t + 1] = (float)cy;
+	T.com[3 * t + 2] = (float)cz;
+}
+
+__global__ void k_tree_terms(TopNodes T, int vs, double4 *__restrict__ na,
+		double4 *__restrict__ nb, int4 *__restrict__ nc, int *__restrict__ up)
+{
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= *T.ntop || T.npart[t] <= vs)
+		return;
+	const int j = T.term[t];
+	na[j] = make_double4(0, 0, 0, 1);   /* terminator: Mass = 1, TOP bit */
+	nb[j] = make_double4(0, 0, 0, 0);
+	nc[j] = make_int4(0, 0, 1 << TDV_BIT_TOP, -1);
+	up[j] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_tree_centres(size_t nnodes,
+		const double *__restrict__ x, const double *__restrict__ y,
+		const double *__restrict__ z, TopNodes T, double dom_size,
+		const int4 *__restrict__ nc, const int *__restrict__ up,
+		double4 *__restrict__ nb, size_t npart, unsigned *err)
+{
+	const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= nnodes)
+		return;
+	const int4 me = nc[j];
+	if (me.w < 0)
+		return;                          /* terminator */
+	int path[TDV_NT + 2];
+	int np = 0;
+	int q = (int)j;
+	int bits = me.z;
+	while (!((bits >> TDV_BIT_TOP) & 1)) {
+		const int u = up[q];
+		if (np >= TDV_NT + 1 || u <= 0 || u > q) {
+			atomicOr(err, TDV_EBIT_RANGE);   /* broken parent links */
+			return;
+		}
+		path[np++] = q;
+		q -= u;
+		bits = nc[q].z;
+	}
+	const int t = up[q];                 /* the root's DUp is its top node */
+	if (t < 0 || t >= *T.ntop) {
+		atomicOr(err, TDV_EBIT_RANGE);
+		return;
+	}
+	double cx = T.pos[3 * t], cy = T.pos[3 * t + 1], cz = T.pos[3 * t + 2];
+	for (int s = np - 1; s >= 0; s--) {
+		const int nd = path[s];
+		const int4 cn = nc[nd];
+		const int lv = cn.z & 63;
+		const int f = cn.w;
+		if (f < 0 || (size_t)f >= npart) {
+			atomicOr(err, TDV_EBIT_RANGE);
+			return;
+		}
+		const double h = int_shift_size(dom_size, lv);
+		cx = cx + (x[f] > cx ? 1 : -1) * h * 0.5;
+		cy = cy + (y[f] > cy ? 1 : -1) * h * 0.5;
+		cz = cz + (z[f] > cz ? 1 : -1) * h * 0.5;
+	}
+	const int lv = me.z & 63;
+	nb[j] = make_double4(cx, cy, cz, dom_size / (double)(1ull << lv));
+}
+
+int tdv_build_tree(tdv_context *c)
+{
+	const size_t n = c->n;
+	const int vs = c->par.vector_size;
+	TDV_HIP(c, hipMemsetAsync(c->minpack, 0xFF, n * sizeof(uint32_t),
+			c->stream));
+	hipLaunchKernelGGL(k_tree_marks, dim3(tdv_blocks(n, 256)), dim3(256), 0,
+			c->stream, n, c->key, c->top, vs, c->minpack, c->d_err);
+	hipLaunchKernelGGL(k_tree_counts, dim3(tdv_blocks(n, 256)), dim3(256), 0,
+			c->stream, n, c->key, c->top, vs, c->minpack,
+			reinterpret_cast<uint32_t *>(c->cnt), c->lohi);
+	TDV_TRY(tdv_scan_u32(c, reinterpret_cast<uint32_t *>(c->cnt), c->off, n));
+	hipLaunchKernelGGL(k_tree_targets, dim3(1), dim3(64), 0, c->stream, c->top,
+			vs, c->off);
+	TDV_HIP(c, hipGetLastError());
+	int nn = 0;
+	TDV_HIP(c, hipMemcpyAsync(&nn, c->top.nnodes, sizeof(int),
+			hipMemcpyDeviceToHost, c->stream));
+	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	TDV_TRY(tdv_check_err_flags(c));
+	TDV_TRY(tdv_ensure_nodes(c, (size_t)nn));
+	c->nnodes = (size_t)nn;
+	if (nn > 0)
+		TDV_HIP(c, hipMemsetAsync(c->node_c, 0, (size_t)nn * sizeof(int4),
+				c->stream));
+	hipLaunchKernelGGL(k_tree_nodes, dim3(tdv_blocks(n, 256)), dim3(256), 0,
+			c->stream, n, c->key, c->pos[0], c->pos[1], c->pos[2], c->mass,
+			c->top, vs, reinterpret_cast<uint32_t *>(c->cnt), c->off, c->lohi,
+			c->node_a, c->node_c, c->node_up, c->tree_parent);
+	hipLaunchKernelGGL(k_tree_terms, dim3(1), dim3(64), 0, c->stream, c->top,
+			vs, c->node_a, c->node_b, c->node_c, c->node_up);
+	hipLaunchKernelGGL(k_tree_small_top, dim3(1), dim3(64), 0, c->stream,
+			c->pos[0], c->pos[1], c->pos[2], c->mass, c->top, vs, c->node_a,
+			c->tree_parent);
+	if (nn > 0)
+		hipLaunchKernelGGL(k_tree_centres, dim3(tdv_blocks((size_t)nn, 256)),
+				dim3(256), 0, c->stream, (size_t)nn, c->pos[0], c->pos[1],
+				c->pos[2], c->top, c->dom_size, c->node_c, c->node_up,
+				c->node_b, n, c->d_err);
+	TDV_HIP(c, hipGetLastError());
+	c->have_tree = 1;
+	return 0;
+}

@@ -1,0 +1,153 @@
+/*
+ * Internal declarations of the MI355X gravity engine: device layouts, the
+ * context, kernel launchers.  gfx950 only.
+ *
+ * HBM layout (all SoA, 256-B aligned allocations):
+ *   particles  pos[3], vel[3], acc[3], gacc[3], mass, last_acc, cost (f64),
+ *              key (2 x u64 per particle), tree_parent, time_bin, id (32-bit)
+ *              plus pm = {x, y, z, m} double4 in PH order for the walk;
+ *   tree       node_a = {CoM, Mass} double4, node_b = {centre, size}
+ *              double4, node_c = {DNext, Npart, Bitfield, first particle}
+ *              int4, node_up = DUp;
+ *   top nodes  <= 64 records (level-2 cells, NTarget = 1);
+ *   Ewald      Fx, Fy, Fz, Fp (65^3 f64 each), divided by Box^2 / Box.
+ */
+#ifndef TDV_INTERNAL_H
+#define TDV_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <string>
+
+#include "../../include/tandav_gpu.h"
+
+#define TDV_NT 42            /* N_PEANO_TRIPLETS (peano.h:8) */
+#define TDV_TOP_LEVEL 2      /* MIN_LEVEL with NTarget = 1 (domain.c:3) */
+#define TDV_NE 64            /* N_EWALD (Gravity/periodic.c:9) */
+#define TDV_NE1 65
+#define TDV_NE3 (TDV_NE1 * TDV_NE1 * TDV_NE1)
+#define TDV_BIT_LOCAL 9
+#define TDV_BIT_TOP 10
+#define TDV_MAX_TOP 64
+
+/* error bits raised by kernels into ctx->d_err */
+#define TDV_EBIT_DUP 1u
+#define TDV_EBIT_DEPTH 2u
+#define TDV_EBIT_RANGE 4u
+#define TDV_EBIT_TIES 8u
+
+struct TopNodes {            /* device arrays of length TDV_MAX_TOP */
+	int *first, *npart, *level, *target, *term;
+	uint64_t *key;
+	float *pos;              /* 3 per node */
+	float *mass;
+	float *com;              /* 3 per node */
+	int *cell2top;           /* level-2 cell id -> top node index or -1 */
+	int *ntop;               /* [1] */
+	int *nnodes;             /* [1] total tree nodes */
+};
+
+struct tdv_context {
+	tdv_params par;
+	double Epsilon, Epsilon2, Epsilon3;
+	double dom_size, dom_origin[3], dom_center[3];
+	hipStream_t stream;
+	std::string err;
+
+	size_t n, cap;                   /* particles, allocated particles */
+	double *pos[3], *vel[3], *acc[3], *gacc[3], *mass, *last_acc, *cost;
+	double *tmp_f64;                 /* reorder double buffer */
+	uint64_t *key;                   /* 2 words per particle */
+	uint64_t *key_tmp;
+	int *tree_parent, *time_bin;
+	uint32_t *id;
+	double4 *pm;                     /* {x, y, z, m} for the walk */
+	int *active;                     /* active list */
+	int nactive;
+
+	/* sort scratch */
+	uint64_t *sk[2];
+	uint32_t *sv[2];
+	uint32_t *hist;
+	size_t hist_cap;
+	uint32_t *scan_part;
+	uint32_t *perm;
+
+	/* build scratch (per particle) */
+	uint32_t *minpack;
+	int *cnt;
+	uint32_t *off;                   /* n+1 exclusive scan of cnt */
+	int *lohi;
+
+	/* tree */
+	size_t node_cap, nnodes;
+	double4 *node_a, *node_b;
+	int4 *node_c;                    /* DNext, Npart, Bitfield, first part. */
+	int *node_up;                    /* DUp */
+
+	TopNodes top;
+	int ntop;
+
+	double *ewald;                   /* 4 * TDV_NE3, normalised */
+	int have_ewald;
+
+	unsigned *d_err;
+	int have_domain, have_tree;
+
+	hipEvent_t ev[16];
+	float stage_ms[8];
+};
+
+/* ---- launchers (tdv_keys_sort.hip) ---- */
+void tdv_launch_keys(tdv_context *c, size_t n, const double *x,
+		const double *y, const double *z, const double o[3], double size,
+		uint64_t *out, int reversed);
+int tdv_sort_device(tdv_context *c, size_t n, const uint64_t *keys,
+		uint32_t *perm_out);
+void tdv_launch_gather_f64(tdv_context *c, size_t n, const uint32_t *perm,
+		const double *in, double *out);
+void tdv_launch_gather_i32(tdv_context *c, size_t n, const uint32_t *perm,
+		const int *in, int *out);
+void tdv_launch_gather_key(tdv_context *c, size_t n, const uint32_t *perm,
+		const uint64_t *in, uint64_t *out);
+void tdv_launch_reverse_keys(tdv_context *c, size_t n, const uint64_t *in,
+		uint64_t *out);
+void tdv_launch_pack_pm(tdv_context *c);
+int tdv_domain_nonperiodic(tdv_context *c);
+int tdv_scan_u32(tdv_context *c, const uint32_t *in, uint32_t *out,
+		size_t n);
+
+/* ---- launchers (tdv_build.hip) ---- */
+int tdv_build_top_nodes(tdv_context *c);
+int tdv_build_tree(tdv_context *c);
+
+/* ---- launchers (tdv_walk64.hip, tdv_ewald.hip) ---- */
+int tdv_walk64(tdv_context *c, int use_bh);
+int tdv_walk64_ewald(tdv_context *c, int use_bh);
+void tdv_launch_safe_last_accel(tdv_context *c);
+int tdv_ewald_table_device(tdv_context *c, double *out4_unnormalised);
+void tdv_launch_ewald_correction(tdv_context *c, size_t n, const double *dr,
+		double *f);
+
+/* ---- helpers (tdv_api.hip) ---- */
+int tdv_fail(tdv_context *c, int code, const std::string &msg);
+int tdv_check(tdv_context *c, hipError_t e, const char *what);
+int tdv_check_err_flags(tdv_context *c);
+int tdv_ensure_nodes(tdv_context *c, size_t nodes);
+
+#define TDV_TRY(expr)                                                       \
+	do {                                                                    \
+		int _r = (expr);                                                    \
+		if (_r)                                                             \
+			return _r;                                                      \
+	} while (0)
+
+#define TDV_HIP(ctx, expr) TDV_TRY(tdv_check((ctx), (expr), #expr))
+
+static inline unsigned tdv_blocks(size_t n, unsigned t)
+{
+	return (unsigned)((n + t - 1) / t);
+}
+
+#endif

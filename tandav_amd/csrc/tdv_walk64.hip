@@ -1,0 +1,492 @@
+/*
+ * FP64 tree walks, one lane per active particle, reproducing the
+ * reference's per-particle decisions and summation order:
+ *   k_walk64        Gravity_Tree_Acceleration (Gravity/tree_accel.c:32-394):
+ *                   top-node loop, gravity_tree_walk (relative criterion +
+ *                   "inside" test) or gravity_tree_walk_BH, interact()
+ *   k_walk64_ewald  Gravity_Tree_Periodic (Gravity/tree_periodic.c:23-450)
+ *                   with Ewald_Correction (Gravity/periodic.c:35-113)
+ * Opening decisions use the same FP64 expressions in the same order as the
+ * reference, so the interaction set and P.Cost are identical; the build is
+ * compiled without FP contraction so every product and sum rounds once.
+ * Lanes of a wavefront are PH neighbours, so their walks visit nearly the
+ * same nodes and the node loads coalesce to the same cache lines.
+ */
+#include "tdv_internal.h"
+#include "tdv_device.h"
+
+struct WalkParams {
+	int ntop, vs, compat, periodic, use_bh;
+	double G, theta_bh, theta_bh2, alpha_rel, eps2, eps3, box, boxhalf;
+	double dom_size, box2ewald;
+};
+
+struct TopShared {
+	float pos[TDV_MAX_TOP * 3], com[TDV_MAX_TOP * 3], mass[TDV_MAX_TOP];
+	int npart[TDV_MAX_TOP], target[TDV_MAX_TOP], first[TDV_MAX_TOP];
+	int level[TDV_MAX_TOP], term[TDV_MAX_TOP];
+};
+
+__device__ __forceinline__ void load_top(TopShared &s, const TopNodes &T,
+		int ntop)
+{
+	for (int t = threadIdx.x; t < ntop; t += blockDim.x) {
+		for (int d = 0; d < 3; d++) {
+			s.pos[3 * t + d] = T.pos[3 * t + d];
+			s.com[3 * t + d] = T.com[3 * t + d];
+		}
+		s.mass[t] = T.mass[t];
+		s.npart[t] = T.npart[t];
+		s.target[t] = T.target[t];
+		s.first[t] = T.first[t];
+		s.level[t] = T.level[t];
+		s.term[t] = T.term[t];
+	}
+	__syncthreads();
+}
+
+/* Periodic_Nearest (periodic.c:62-80) */
+__device__ __forceinline__ double wrap(double d, const WalkParams &w)
+{
+	if (!w.periodic)
+		return d;
+	if (d > w.boxhalf)
+		d -= w.box;
+	else if (d < -w.boxhalf)
+		d += w.box;
+	return d;
+}
+
+struct Acc {
+	double x, y, z, cost;
+};
+
+/* interact (Gravity/tree_accel.c:362-394): Dehnen K1 kernel, h < 0 */
+__device__ __forceinline__ void interact(Acc &a, double m, double dx,
+		double dy, double dz, double r2, const WalkParams &w)
+{
+	double fac = w.G * m;
+	if (r2 < w.eps2) {
+		const double u2 = r2 / w.eps2;
+		fac *= (175 - u2 * (294 - u2 * 135)) / (16 * w.eps3);
+	} else {
+		const double r_inv = 1 / sqrt(r2);
+		fac *= r_inv * r_inv * r_inv;
+	}
+	a.x += fac * dx;
+	a.y += fac * dy;
+	a.z += fac * dz;
+	a.cost += 1;
+}
+
+__device__ __forceinline__ void pp_range(Acc &a, const double4 *pm, int first,
+		int last, double px, double py, double pz, const WalkParams &w)
+{
+	for (int j = first; j < last; j++) {
+		const double4 q = pm[j];
+		const double dx = wrap(q.x - px, w), dy = wrap(q.y - py, w),
+				dz = wrap(q.z - pz, w);
+		const double r2 = dx * dx + dy * dy + dz * dz;
+		if (r2 != 0)
+			interact(a, q.w, dx, dy, dz, r2, w);
+	}
+}
+
+__global__ void __launch_bounds__(256) k_walk64(int nact,
+		const int *__restrict__ active, const double4 *__restrict__ pm,
+		const double *__restrict__ last_acc, TopNodes T,
+		const double4 *__restrict__ na, const double4 *__restrict__ nb,
+		const int4 *__restrict__ nc, WalkParams w, double *__restrict__ ax,
+		double *__restrict__ ay, double *__restrict__ az,
+		double *__restrict__ gx, double *__restrict__ gy,
+		double *__restrict__ gz, double *__restrict__ cost, int npart_total,
+		unsigned *err)
+{
+	__shared__ TopShared S;
+	load_top(S, T, w.ntop);
+	const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+	if (gid >= nact)
+		return;
+	const int ip = active[gid];
+	const double4 p = pm[ip];
+	const double fac = last_acc[ip] / w.G * w.alpha_rel;
+	Acc a = { 0, 0, 0, 0 };
+
+	for (int j = 0; j < w.ntop; j++) {
+		/* interact_with_topnode, Gravity/tree_accel.c:137-176 */
+		const double nSize = w.dom_size / ((double)(1ull << S.level[j]));
+		{
+			const double dx = (double)S.pos[3 * j] - p.x;
+			const double dy = (double)S.pos[3 * j + 1] - p.y;
+			const double dz = (double)S.pos[3 * j + 2] - p.z;
+			const bool inside = fabs(dx) < 0.6 * nSize && fabs(dy) < 0.6 * nSize
+					&& fabs(dz) < 0.6 * nSize;
+			if (!inside) {
+				const double rx = wrap((double)S.com[3 * j] - p.x, w);
+				const double ry = wrap((double)S.com[3 * j + 1] - p.y, w);
+				const double rz = wrap((double)S.com[3 * j + 2] - p.z, w);
+				const double r2 = rx * rx + ry * ry + rz * rz;
+				const double m = (double)S.mass[j];
+				const bool open = w.use_bh ? (nSize * nSize > r2 * w.theta_bh)
+						: (m * nSize * nSize > r2 * r2 * fac);
+				if (!open) {
+					interact(a, m, rx, ry, rz, r2, w);
+					continue;
+				}
+			}
+		}
+		if (S.npart[j] <= w.vs) {
+			/* interact_with_topnode_particles, tree_accel.c:183-203 */
+			const int f = w.compat ? S.target[j] : S.first[j];
+			if (f < 0 || f + S.npart[j] > npart_total) {
+				atomicOr(err, TDV_EBIT_RANGE);
+				continue;
+			}
+			pp_range(a, pm, f, f + S.npart[j], p.x, p.y, p.z, w);
+			continue;
+		}
+		/* gravity_tree_walk(_BH), Gravity/tree_accel.c:211-353 */
+		const int start = S.target[j], term = S.term[j];
+		int node = start;
+		for (;;) {
+			if (node < start || node > term) {   /* never for a valid tree */
+				atomicOr(err, TDV_EBIT_RANGE);
+				break;
+			}
+			const int4 c = nc[node];
+			if (c.x == 0) {                      /* the terminator */
+				if (node == start)
+					atomicOr(err, TDV_EBIT_RANGE);
+				break;
+			}
+			if (c.x < 0) {
+				const int f = -c.x - 1;
+				if (f + c.y > npart_total) {
+					atomicOr(err, TDV_EBIT_RANGE);
+					break;
+				}
+				pp_range(a, pm, f, f + c.y, p.x, p.y, p.z, w);
+				node++;
+				continue;
+			}
+			const double4 A = na[node];
+			const double rx = wrap(A.x - p.x, w), ry = wrap(A.y - p.y, w),
+					rz = wrap(A.z - p.z, w);
+			const double r2 = rx * rx + ry * ry + rz * rz;
+			const double m = A.w;
+			const double4 B = nb[node];
+			const double s = B.w;
+			if (w.use_bh) {
+				if (s * s > r2 * w.theta_bh) {
+					node++;
+					continue;
+				}
+			} else {
+				if (m * s * s > r2 * r2 * fac) {
+					node++;
+					continue;
+				}
+				const double sx = wrap(B.x - p.x, w), sy = wrap(B.y - p.y, w),
+						sz = wrap(B.z - p.z, w);
+				if (fabs(sx) < 0.6 * s && fabs(sy) < 0.6 * s
+						&& fabs(sz) < 0.6 * s) {
+					node++;
+					continue;
+				}
+			}
+			interact(a, m, rx, ry, rz, r2, w);
+			node += c.x;
+		}
+	}
+	/* add_recv_to, Gravity/tree_accel.c:110-129 */
+	ax[ip] = ax[ip] + a.x;
+	ay[ip] = ay[ip] + a.y;
+	az[ip] = az[ip] + a.z;
+	gx[ip] = a.x;
+	gy[ip] = a.y;
+	gz[ip] = a.z;
+	cost[ip] = a.cost;
+}
+
+/* Ewald_Correction, Gravity/periodic.c:35-113 (table already / Box^2) */
+__device__ __forceinline__ void ewald_corr(const double *__restrict__ tab,
+		double dx, double dy, double dz, double box2ewald, double f[3])
+{
+	double sg[3] = { -1, -1, -1 };
+	if (dx < 0) {
+		dx = -dx;
+		sg[0] = 1;
+	}
+	if (dy < 0) {
+		dy = -dy;
+		sg[1] = 1;
+	}
+	if (dz < 0) {
+		dz = -dz;
+		sg[2] = 1;
+	}
+	double u = dx * box2ewald, v = dy * box2ewald, q = dz * box2ewald;
+	int i = (int)u, j = (int)v, k = (int)q;
+	if (i >= TDV_NE)
+		i = TDV_NE - 1;
+	if (j >= TDV_NE)
+		j = TDV_NE - 1;
+	if (k >= TDV_NE)
+		k = TDV_NE - 1;
+	u -= i;
+	v -= j;
+	q -= k;
+	const double w0 = (1 - u) * (1 - v) * (1 - q), w1 = (1 - u) * (1 - v) * q,
+			w2 = (1 - u) * v * (1 - q), w3 = (1 - u) * v * q,
+			w4 = u * (1 - v) * (1 - q), w5 = u * (1 - v) * q,
+			w6 = u * v * (1 - q), w7 = u * v * q;
+	const int b = (i * TDV_NE1 + j) * TDV_NE1 + k;
+	const int di = TDV_NE1 * TDV_NE1, dj = TDV_NE1;
+#pragma unroll
+	for (int d = 0; d < 3; d++) {
+		const double *F = tab + d * TDV_NE3 + b;
+		f[d] = (w0 * F[0] + w1 * F[1] + w2 * F[dj] + w3 * F[dj + 1]
+				+ w4 * F[di] + w5 * F[di + 1] + w6 * F[di + dj]
+				+ w7 * F[di + dj + 1]) * sg[d];
+	}
+}
+
+__device__ __forceinline__ void ewald_cube(Acc &a, const double *tab,
+		double dx, double dy, double dz, double m, const WalkParams &w)
+{
+	double f[3];
+	ewald_corr(tab, dx, dy, dz, w.box2ewald, f);
+	a.x += w.G * m * f[0];
+	a.y += w.G * m * f[1];
+	a.z += w.G * m * f[2];
+	a.cost += 1;
+}
+
+__device__ __forceinline__ void ewald_pp_range(Acc &a, const double *tab,
+		const double4 *pm, int first, int last, double px, double py,
+		double pz, const WalkParams &w)
+{
+	for (int j = first; j < last; j++) {
+		const double4 q = pm[j];
+		const double dx = wrap(q.x - px, w), dy = wrap(q.y - py, w),
+				dz = wrap(q.z - pz, w);
+		if (dx != 0 && dy != 0 && dz != 0)
+			ewald_cube(a, tab, dx, dy, dz, q.w, w);
+	}
+}
+
+__global__ void __launch_bounds__(256) k_walk64_ewald(int nact,
+		const int *__restrict__ active, const double4 *__restrict__ pm,
+		const double *__restrict__ last_acc, TopNodes T,
+		const double4 *__restrict__ na, const double4 *__restrict__ nb,
+		const int4 *__restrict__ nc, const double *__restrict__ tab,
+		WalkParams w, double *__restrict__ ax, double *__restrict__ ay,
+		double *__restrict__ az, double *__restrict__ gx,
+		double *__restrict__ gy, double *__restrict__ gz,
+		double *__restrict__ cost, int npart_total, unsigned *err)
+{
+	__shared__ TopShared S;
+	load_top(S, T, w.ntop);
+	const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+	if (gid >= nact)
+		return;
+	const int ip = active[gid];
+	const double4 p = pm[ip];
+	const double fac = last_acc[ip] / w.G * w.alpha_rel;
+	Acc a = { 0, 0, 0, 0 };
+
+	for (int j = 0; j < w.ntop; j++) {
+		/* interact_with_topnode, Gravity/tree_periodic.c:112-177 */
+		const double s = w.dom_size / (double)(1ull << S.level[j]);
+		bool want_open = false;
+		const double rx = wrap((double)S.com[3 * j] - p.x, w);
+		const double ry = wrap((double)S.com[3 * j + 1] - p.y, w);
+		const double rz = wrap((double)S.com[3 * j + 2] - p.z, w);
+		const double r2 = rx * rx + ry * ry + rz * rz;
+		if (w.use_bh) {
+			if (s * s > r2 * w.theta_bh)
+				want_open = true;
+		} else {
+			const double m = (double)S.mass[j];
+			if (m * s * s > r2 * r2 * fac)
+				want_open = true;
+		}
+		if (fabs(p.x - (double)S.pos[3 * j]) < 0.6 * s
+				&& fabs(p.y - (double)S.pos[3 * j + 1]) < 0.6 * s
+				&& fabs(p.z - (double)S.pos[3 * j + 2]) < 0.6 * s)
+			want_open = true;
+		/* a node that wants opening is walked only when it straddles the
+		 * half box or is larger than 0.2 box; otherwise the correction of
+		 * its centre of mass is used */
+		bool walk = false;
+		if (want_open) {
+			const double lim = 0.5 * (w.box - s);
+			walk = fabs(rx) > lim || fabs(ry) > lim || fabs(rz) > lim
+					|| s > 0.2 * w.box;
+		}
+		if (!walk) {
+			ewald_cube(a, tab, rx, ry, rz, (double)S.mass[j], w);
+			continue;
+		}
+		if (S.npart[j] <= w.vs) {
+			const int f = w.compat ? S.target[j] : S.first[j];
+			if (f < 0 || f + S.npart[j] > npart_total) {
+				atomicOr(err, TDV_EBIT_RANGE);
+				continue;
+			}
+			ewald_pp_range(a, tab, pm, f, f + S.npart[j], p.x, p.y, p.z, w);
+			continue;
+		}
+		/* gravity_tree_walk_ewald(_BH), Gravity/tree_periodic.c:216-422 */
+		const int start = S.target[j];
+		const int end = start + nc[start].x;
+		if (end != S.term[j]) {
+			atomicOr(err, TDV_EBIT_RANGE);
+			continue;
+		}
+		int node = start;
+		while (node != end) {
+			if (node < start || node > end) {    /* never for a valid tree */
+				atomicOr(err, TDV_EBIT_RANGE);
+				break;
+			}
+			const int4 c = nc[node];
+			if (c.x < 0) {
+				const int f = -c.x - 1;
+				if (f + c.y > npart_total) {
+					atomicOr(err, TDV_EBIT_RANGE);
+					break;
+				}
+				ewald_pp_range(a, tab, pm, f, f + c.y, p.x, p.y, p.z, w);
+				node++;
+				continue;
+			}
+			const double4 A = na[node];
+			const double dx = wrap(A.x - p.x, w), dy = wrap(A.y - p.y, w),
+					dz = wrap(A.z - p.z, w);
+			const double d2 = dx * dx + dy * dy + dz * dz;
+			const double m = A.w;
+			const double4 B = nb[node];
+			const double ns = B.w;
+			const double sx = B.x - p.x, sy = B.y - p.y, sz = B.z - p.z;
+			bool open = w.use_bh ? (ns * ns > d2 * w.theta_bh2)
+					: (m * ns * ns > d2 * d2 * fac);
+			if (!open && fabs(sx) < 0.6 * ns && fabs(sy) < 0.6 * ns
+					&& fabs(sz) < 0.6 * ns)
+				open = true;
+			if (open) {
+				const double lim = 0.5 * (w.box - ns);
+				if (fabs(wrap(sx, w)) > lim || fabs(wrap(sy, w)) > lim
+						|| fabs(wrap(sz, w)) > lim || ns > 0.2 * w.box) {
+					node++;
+					continue;
+				}
+			}
+			ewald_cube(a, tab, dx, dy, dz, m, w);
+			node += c.x;
+		}
+	}
+	/* add_recv_to, Gravity/tree_periodic.c:91-110 */
+	ax[ip] = ax[ip] + a.x;
+	ay[ip] = ay[ip] + a.y;
+	az[ip] = az[ip] + a.z;
+	gx[ip] = gx[ip] + a.x;
+	gy[ip] = gy[ip] + a.y;
+	gz[ip] = gz[ip] + a.z;
+	cost[ip] = cost[ip] + a.cost;
+}
+
+static WalkParams walk_params(const tdv_context *c, int use_bh)
+{
+	WalkParams w;
+	w.ntop = c->ntop;
+	w.vs = c->par.vector_size;
+	w.compat = c->par.compat;
+	w.periodic = c->par.periodic;
+	w.use_bh = use_bh;
+	w.G = c->par.gravity;
+	w.theta_bh = c->par.theta_bh;
+	w.theta_bh2 = c->par.theta_bh * c->par.theta_bh;
+	w.alpha_rel = c->par.alpha_rel;
+	w.eps2 = c->Epsilon2;
+	w.eps3 = c->Epsilon3;
+	w.box = c->par.boxsize;
+	w.boxhalf = c->par.boxsize / 2.0;
+	w.dom_size = c->dom_size;
+	w.box2ewald = 2 * TDV_NE / c->par.boxsize;
+	return w;
+}
+
+int tdv_walk64(tdv_context *c, int use_bh)
+{
+	if (!c->nactive)
+		return 0;
+	const WalkParams w = walk_params(c, use_bh);
+	hipLaunchKernelGGL(k_walk64, dim3(tdv_blocks((size_t)c->nactive, 256)),
+			dim3(256), 0, c->stream, c->nactive, c->active, c->pm, c->last_acc,
+			c->top, c->node_a, c->node_b, c->node_c, w, c->acc[0], c->acc[1],
+			c->acc[2], c->gacc[0], c->gacc[1], c->gacc[2], c->cost, (int)c->n,
+			c->d_err);
+	return tdv_check(c, hipGetLastError(), "k_walk64");
+}
+
+int tdv_walk64_ewald(tdv_context *c, int use_bh)
+{
+	if (!c->nactive)
+		return 0;
+	if (!c->have_ewald)
+		return tdv_fail(c, TDV_ERR_ARG, "periodic walk without an Ewald table");
+	const WalkParams w = walk_params(c, use_bh);
+	hipLaunchKernelGGL(k_walk64_ewald,
+			dim3(tdv_blocks((size_t)c->nactive, 256)), dim3(256), 0, c->stream,
+			c->nactive, c->active, c->pm, c->last_acc, c->top, c->node_a,
+			c->node_b, c->node_c, c->ewald, w, c->acc[0], c->acc[1], c->acc[2],
+			c->gacc[0], c->gacc[1], c->gacc[2], c->cost, (int)c->n, c->d_err);
+	return tdv_check(c, hipGetLastError(), "k_walk64_ewald");
+}
+
+/* safe_last_accel, accel.c:28-43 */
+__global__ void k_safe_last_accel(int nact, const int *__restrict__ active,
+		double *__restrict__ ax, double *__restrict__ ay,
+		double *__restrict__ az, double *__restrict__ last)
+{
+	const int g = blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= nact)
+		return;
+	const int i = active[g];
+	last[i] = sqrt(ax[i] * ax[i] + ay[i] * ay[i] + az[i] * az[i]);
+	ax[i] = 0;
+	ay[i] = 0;
+	az[i] = 0;
+}
+
+void tdv_launch_safe_last_accel(tdv_context *c)
+{
+	if (!c->nactive)
+		return;
+	hipLaunchKernelGGL(k_safe_last_accel,
+			dim3(tdv_blocks((size_t)c->nactive, 256)), dim3(256), 0, c->stream,
+			c->nactive, c->active, c->acc[0], c->acc[1], c->acc[2],
+			c->last_acc);
+}
+
+__global__ void k_ewald_correction_batch(size_t n, const double *__restrict__ dr,
+		double *__restrict__ f, const double *__restrict__ tab, double b2e)
+{
+	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	double r[3];
+	ewald_corr(tab, dr[3 * i], dr[3 * i + 1], dr[3 * i + 2], b2e, r);
+	f[3 * i] = r[0];
+	f[3 * i + 1] = r[1];
+	f[3 * i + 2] = r[2];
+}
+
+void tdv_launch_ewald_correction(tdv_context *c, size_t n, const double *dr,
+		double *f)
+{
+	hipLaunchKernelGGL(k_ewald_correction_batch, dim3(tdv_blocks(n, 256)),
+			dim3(256), 0, c->stream, n, dr, f, c->ewald,
+			2 * TDV_NE / c->par.boxsize);
+}

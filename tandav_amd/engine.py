@@ -1,0 +1,308 @@
+"""ctypes binding of libtandav_gpu.so and the reference-shaped host API.
+
+Method names follow the reference functions they replace; each docstring
+cites the reference file:line.  Arrays cross the boundary as plain host
+pointers and sizes (numpy float64 / int32 / uint32 / uint64 buffers).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtandav_gpu.so")
+
+NODE_DTYPE = np.dtype([("DNext", "<i4"), ("Bitfield", "<u4"), ("DUp", "<i4"),
+                       ("Npart", "<i4"), ("Pos", "<f8", 3), ("Mass", "<f8"),
+                       ("CoM", "<f8", 3), ("Dp", "<f8", 3)])
+TOP_DTYPE = np.dtype([("Key", "<u8"), ("Target", "<i4"), ("Level", "<i4"),
+                      ("First_Part", "<i4"), ("Npart", "<i4"),
+                      ("Pos", "<f4", 3), ("Mass", "<f4"), ("CoM", "<f4", 3),
+                      ("Dp", "<f4", 3)])
+
+STAGES = ("keys", "sort", "reorder", "top_nodes", "build", "walk", "ewald",
+          "kick")
+
+
+class TDVError(RuntimeError):
+    pass
+
+
+class TdvParams(ctypes.Structure):
+    _fields_ = [("periodic", ctypes.c_int), ("boxsize", ctypes.c_double),
+                ("gravity", ctypes.c_double), ("softening", ctypes.c_double),
+                ("theta_bh", ctypes.c_double), ("alpha_rel", ctypes.c_double),
+                ("vector_size", ctypes.c_int), ("compat", ctypes.c_int),
+                ("device", ctypes.c_int)]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+_ctx = ctypes.c_void_p
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load the HIP engine; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise TDVError(
+            "libtandav_gpu.so is missing: build it with "
+            "`python -c 'import __graft_entry__ as g; g.build()'` "
+            "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+    L = ctypes.CDLL(path)
+    L.tdv_create.argtypes = [ctypes.POINTER(TdvParams), ctypes.POINTER(_ctx)]
+    L.tdv_destroy.argtypes = [_ctx]
+    L.tdv_destroy.restype = None
+    L.tdv_last_error.argtypes = [_ctx]
+    L.tdv_last_error.restype = ctypes.c_char_p
+    L.tdv_set_particles.argtypes = [_ctx, _sz, ctypes.POINTER(_dp),
+                                    ctypes.POINTER(_dp), _dp,
+                                    ctypes.POINTER(_dp), _dp, _vp, _vp]
+    L.tdv_get_particles.argtypes = [_ctx, ctypes.POINTER(_dp),
+                                    ctypes.POINTER(_dp), _dp,
+                                    ctypes.POINTER(_dp), ctypes.POINTER(_dp),
+                                    _dp, _dp, _vp, _vp, _vp, _vp]
+    for f in ("tdv_setup_domain", "tdv_domain_decomposition",
+              "tdv_tree_build"):
+        getattr(L, f).argtypes = [_ctx]
+    L.tdv_tree_acceleration.argtypes = [_ctx, ctypes.c_int]
+    L.tdv_compute_acceleration.argtypes = [_ctx, ctypes.c_int]
+    L.tdv_peano_keys.argtypes = [_ctx, _sz, _dp, _dp, _dp, _dp,
+                                 ctypes.c_double, ctypes.c_int, _vp]
+    L.tdv_sort_keys.argtypes = [_ctx, _sz, _vp, _vp]
+    L.tdv_ewald_correction.argtypes = [_ctx, _sz, _dp, _dp]
+    L.tdv_get_ewald_table.argtypes = [_ctx, _dp]
+    L.tdv_set_ewald_table.argtypes = [_ctx, _dp]
+    L.tdv_get_domain.argtypes = [_ctx, _dp, _dp, _dp]
+    L.tdv_get_top_nodes.argtypes = [_ctx, _vp, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_int)]
+    L.tdv_get_tree.argtypes = [_ctx, _vp, _sz, ctypes.POINTER(_sz)]
+    L.tdv_stage_times.argtypes = [_ctx, ctypes.POINTER(ctypes.c_float)]
+    L.tdv_num_particles.argtypes = [_ctx]
+    L.tdv_num_particles.restype = _sz
+    _lib = L
+    return L
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _col3(cols):
+    arr = (_dp * 3)()
+    for d in range(3):
+        arr[d] = _d(cols[d]) if cols is not None else None
+    return arr
+
+
+class Engine:
+    """One GPU's force path with the reference's calling sequence."""
+
+    def __init__(self, periodic=False, boxsize=1.0, gravity=1.0,
+                 softening=0.01, theta_bh=0.1, alpha_rel=0.02, vector_size=64,
+                 compat=True, device=0, ewald_table=None):
+        self.lib = load_library()
+        self.params = TdvParams(int(periodic), float(boxsize), float(gravity),
+                                float(softening), float(theta_bh),
+                                float(alpha_rel), int(vector_size),
+                                int(compat), int(device))
+        self._c = _ctx()
+        rc = self.lib.tdv_create(ctypes.byref(self.params),
+                                 ctypes.byref(self._c))
+        if rc:
+            raise TDVError("tdv_create failed (%d)" % rc)
+        self.periodic = periodic
+        self.n = 0
+        if ewald_table is not None:
+            self.set_ewald_table(ewald_table)
+
+    def close(self):
+        if getattr(self, "_c", None) and self._c.value:
+            self.lib.tdv_destroy(self._c)
+            self._c = _ctx()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ok(self, rc, what):
+        if rc:
+            msg = self.lib.tdv_last_error(self._c).decode()
+            raise TDVError("%s: error %d: %s" % (what, rc, msg))
+
+    # -- particle columns (particles.h:17-37) --------------------------------
+    def set_particles(self, pos, mass, vel=None, acc=None, last_acc_mag=None,
+                      time_bin=None, ids=None):
+        n = pos.shape[0]
+        p = [np.ascontiguousarray(pos[:, d], dtype=np.float64)
+             for d in range(3)]
+        v = None if vel is None else [
+            np.ascontiguousarray(vel[:, d], dtype=np.float64)
+            for d in range(3)]
+        a = None if acc is None else [
+            np.ascontiguousarray(acc[:, d], dtype=np.float64)
+            for d in range(3)]
+        m = np.ascontiguousarray(mass, dtype=np.float64)
+        la = (None if last_acc_mag is None
+              else np.ascontiguousarray(last_acc_mag, dtype=np.float64))
+        tb = (None if time_bin is None
+              else np.ascontiguousarray(time_bin, dtype=np.int32))
+        ii = None if ids is None else np.ascontiguousarray(ids,
+                                                           dtype=np.uint32)
+        rc = self.lib.tdv_set_particles(
+            self._c, n, _col3(p), _col3(v) if v else None, _d(m),
+            _col3(a) if a else None, _d(la) if la is not None else None,
+            tb.ctypes.data_as(_vp) if tb is not None else None,
+            ii.ctypes.data_as(_vp) if ii is not None else None)
+        self._ok(rc, "tdv_set_particles")
+        self.n = n
+
+    def get(self, *names):
+        """Columns by name: pos vel mass acc grav_acc cost last_acc_mag
+        tree_parent key id time_bin."""
+        n = self.n
+        want = set(names) if names else {
+            "pos", "vel", "mass", "acc", "grav_acc", "cost", "last_acc_mag",
+            "tree_parent", "key", "id", "time_bin"}
+        out = {}
+        cols = {}
+        for k in ("pos", "vel", "acc", "grav_acc"):
+            if k in want:
+                cols[k] = [np.zeros(n) for _ in range(3)]
+        one = {}
+        for k in ("mass", "cost", "last_acc_mag"):
+            if k in want:
+                one[k] = np.zeros(n)
+        tp = np.zeros(n, np.int32) if "tree_parent" in want else None
+        key = np.zeros((n, 2), np.uint64) if "key" in want else None
+        ids = np.zeros(n, np.uint32) if "id" in want else None
+        tb = np.zeros(n, np.int32) if "time_bin" in want else None
+        rc = self.lib.tdv_get_particles(
+            self._c, _col3(cols["pos"]) if "pos" in cols else None,
+            _col3(cols["vel"]) if "vel" in cols else None,
+            _d(one["mass"]) if "mass" in one else None,
+            _col3(cols["acc"]) if "acc" in cols else None,
+            _col3(cols["grav_acc"]) if "grav_acc" in cols else None,
+            _d(one["cost"]) if "cost" in one else None,
+            _d(one["last_acc_mag"]) if "last_acc_mag" in one else None,
+            tp.ctypes.data_as(_vp) if tp is not None else None,
+            key.ctypes.data_as(_vp) if key is not None else None,
+            ids.ctypes.data_as(_vp) if ids is not None else None,
+            tb.ctypes.data_as(_vp) if tb is not None else None)
+        self._ok(rc, "tdv_get_particles")
+        for k, v in cols.items():
+            out[k] = np.stack(v, 1)
+        out.update(one)
+        if tp is not None:
+            out["tree_parent"] = tp
+        if key is not None:
+            out["key"] = key
+        if ids is not None:
+            out["id"] = ids
+        if tb is not None:
+            out["time_bin"] = tb
+        return out
+
+    # -- the force path --------------------------------------------------------
+    def setup_domain(self):
+        """Setup_Domain_Decomposition (domain.c:151-189)."""
+        self._ok(self.lib.tdv_setup_domain(self._c), "tdv_setup_domain")
+
+    def domain_decomposition(self):
+        """Domain_Decomposition (domain.c:67-145)."""
+        self._ok(self.lib.tdv_domain_decomposition(self._c),
+                 "tdv_domain_decomposition")
+
+    def tree_build(self):
+        """Gravity_Tree_Build (Gravity/tree_build.c:47-122)."""
+        self._ok(self.lib.tdv_tree_build(self._c), "tdv_tree_build")
+
+    def tree_acceleration(self, use_bh):
+        """Gravity_Tree_Acceleration (+ Gravity_Tree_Periodic)."""
+        self._ok(self.lib.tdv_tree_acceleration(self._c, int(use_bh)),
+                 "tdv_tree_acceleration")
+
+    def compute_acceleration(self, prepare_step=True):
+        """Compute_Acceleration (accel.c:15-71)."""
+        self._ok(self.lib.tdv_compute_acceleration(self._c, int(prepare_step)),
+                 "tdv_compute_acceleration")
+
+    # -- element-wise functions -------------------------------------------------
+    def peano_keys(self, x, y, z, origin, size, reversed_=False):
+        """Peano_Key / Reversed_Peano_Key (peano.c:122-276)."""
+        x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        out = np.zeros((x.size, 2), np.uint64)
+        self._ok(self.lib.tdv_peano_keys(self._c, x.size, _d(x), _d(y), _d(z),
+                                         _d(o), float(size), int(reversed_),
+                                         out.ctypes.data_as(_vp)),
+                 "tdv_peano_keys")
+        return out
+
+    def sort_keys(self, keys2):
+        """Qsort_Index(perm, keys, n, 16, cmp_peanoKeys) (sort.c:90-116)."""
+        k = np.ascontiguousarray(keys2, dtype=np.uint64)
+        perm = np.zeros(k.shape[0], np.uint32)
+        self._ok(self.lib.tdv_sort_keys(self._c, k.shape[0],
+                                        k.ctypes.data_as(_vp),
+                                        perm.ctypes.data_as(_vp)),
+                 "tdv_sort_keys")
+        return perm
+
+    def ewald_correction(self, dr):
+        """Ewald_Correction (Gravity/periodic.c:35-113), batched."""
+        dr = np.ascontiguousarray(dr, dtype=np.float64).reshape(-1, 3)
+        f = np.zeros_like(dr)
+        self._ok(self.lib.tdv_ewald_correction(self._c, dr.shape[0], _d(dr),
+                                               _d(f)), "tdv_ewald_correction")
+        return f
+
+    def ewald_table(self):
+        t = np.zeros((4, 65, 65, 65))
+        self._ok(self.lib.tdv_get_ewald_table(self._c, _d(t)),
+                 "tdv_get_ewald_table")
+        return t
+
+    def set_ewald_table(self, t):
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        self._ok(self.lib.tdv_set_ewald_table(self._c, _d(t)),
+                 "tdv_set_ewald_table")
+
+    # -- views --------------------------------------------------------------------
+    @property
+    def domain(self):
+        s = np.zeros(1)
+        o = np.zeros(3)
+        c = np.zeros(3)
+        self._ok(self.lib.tdv_get_domain(self._c, _d(s), _d(o), _d(c)),
+                 "tdv_get_domain")
+        return float(s[0]), o, c
+
+    def top_nodes(self):
+        buf = np.zeros(64, dtype=TOP_DTYPE)
+        n = ctypes.c_int()
+        self._ok(self.lib.tdv_get_top_nodes(self._c, buf.ctypes.data_as(_vp),
+                                            64, ctypes.byref(n)),
+                 "tdv_get_top_nodes")
+        return buf[:n.value].copy()
+
+    def tree(self):
+        n = _sz()
+        self._ok(self.lib.tdv_get_tree(self._c, None, 0, ctypes.byref(n)),
+                 "tdv_get_tree")
+        buf = np.zeros(n.value, dtype=NODE_DTYPE)
+        self._ok(self.lib.tdv_get_tree(self._c, buf.ctypes.data_as(_vp),
+                                       n.value, ctypes.byref(n)),
+                 "tdv_get_tree")
+        return buf
+
+    def stage_times(self):
+        ms = (ctypes.c_float * 8)()
+        self._ok(self.lib.tdv_stage_times(self._c, ms), "tdv_stage_times")
+        return dict(zip(STAGES, list(ms)))
