@@ -390,6 +390,26 @@ __global__ void k_fix_ties(size_t n, const uint64_t *__restrict__ shi,
 			atomicOr(err, TDV_EBIT_DUP);
 }
 
+/* gather the high words of 128-bit keys by a permutation */
+__global__ void k_gather_hi(size_t n, const uint32_t *__restrict__ perm,
+		const uint64_t *__restrict__ keys, uint64_t *__restrict__ out)
+{
+	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		out[i] = keys[2 * (size_t)perm[i] + 1];
+}
+
+__global__ void k_check_dups(size_t n, const uint32_t *__restrict__ perm,
+		const uint64_t *__restrict__ keys, unsigned *err)
+{
+	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i == 0 || i >= n)
+		return;
+	const size_t a = perm[i - 1], b = perm[i];
+	if (keys[2 * a] == keys[2 * b] && keys[2 * a + 1] == keys[2 * b + 1])
+		atomicOr(err, TDV_EBIT_DUP);
+}
+
 int tdv_sort_device(tdv_context *c, size_t n, const uint64_t *keys,
 		uint32_t *perm_out)
 {
@@ -399,6 +419,27 @@ int tdv_sort_device(tdv_context *c, size_t n, const uint64_t *keys,
 	TDV_TRY(radix_sort_u64(c, n, keys, 2, 1, nullptr, &res));
 	hipLaunchKernelGGL(k_fix_ties, dim3(tdv_blocks(n, 256)), dim3(256), 0,
 			c->stream, n, c->sk[res], c->sv[res], keys, c->d_err);
+	unsigned e = 0;
+	TDV_HIP(c, hipMemcpyAsync(&e, c->d_err, sizeof(e), hipMemcpyDeviceToHost,
+			c->stream));
+	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	if (e & TDV_EBIT_TIES) {
+		/* long runs of equal high words: full 128-bit LSD sort, low word
+		 * first (8 passes), then the high words in that order (8 passes) */
+		TDV_HIP(c, hipMemsetAsync(c->d_err, 0, sizeof(unsigned), c->stream));
+		int r1;
+		TDV_TRY(radix_sort_u64(c, n, keys, 2, 0, nullptr, &r1));
+		uint64_t *hi = reinterpret_cast<uint64_t *>(c->tmp_f64);
+		uint32_t *p1 = c->perm == perm_out ? reinterpret_cast<uint32_t *>(c->lohi)
+				: c->perm;
+		TDV_HIP(c, hipMemcpyAsync(p1, c->sv[r1], n * sizeof(uint32_t),
+				hipMemcpyDeviceToDevice, c->stream));
+		hipLaunchKernelGGL(k_gather_hi, dim3(tdv_blocks(n, 256)), dim3(256), 0,
+				c->stream, n, p1, keys, hi);
+		TDV_TRY(radix_sort_u64(c, n, hi, 1, 0, p1, &res));
+		hipLaunchKernelGGL(k_check_dups, dim3(tdv_blocks(n, 256)), dim3(256), 0,
+				c->stream, n, c->sv[res], keys, c->d_err);
+	}
 	TDV_HIP(c, hipMemcpyAsync(perm_out, c->sv[res], n * sizeof(uint32_t),
 			hipMemcpyDeviceToDevice, c->stream));
 	return tdv_check(c, hipGetLastError(), "sort");

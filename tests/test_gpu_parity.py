@@ -67,7 +67,11 @@ def test_sort_rejects_duplicate_keys():
 def test_ewald_table_and_lookup(ewald):
     e = _engine(periodic=True, boxsize=2.0)
     t = e.ewald_table()
-    np.testing.assert_allclose(t, ewald, rtol=1e-12, atol=1e-14)
+    # device erfc/exp/sincos vs glibc differ by an ulp here and there; the
+    # entries next to the origin (r = 1/128) are x/r^3 ~ 1.6e4 minus an
+    # n = 0 term of the same size, so one ulp of 1.6e4 (3.6e-12) survives
+    # the cancellation as an absolute difference
+    np.testing.assert_allclose(t, ewald, rtol=1e-12, atol=1e-11)
     e.set_ewald_table(ewald)
     g = np.random.Generator(np.random.Philox(23))
     dr = g.uniform(-1.0, 1.0, (50000, 3))
