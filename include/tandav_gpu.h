@@ -139,6 +139,11 @@ int tdv_get_tree(tdv_context *ctx, void *out, size_t max, size_t *nnodes);
  * 6 Ewald walk, 7 kick/drift. */
 int tdv_stage_times(tdv_context *ctx, float ms[8]);
 
+/* Sum of P.Cost over the active particles of the last walk, split into
+ * the tree walk (interact, Gravity/tree_accel.c:391) and the Ewald walk
+ * (interact_with_ewald_cube, Gravity/tree_periodic.c:437). */
+int tdv_interaction_counts(tdv_context *ctx, uint64_t counts[2]);
+
 /* Number of particles currently on the device. */
 size_t tdv_num_particles(const tdv_context *ctx);
 

@@ -241,6 +241,8 @@ int tdv_create(const tdv_params *p, tdv_context **out)
 	if (!r)
 		r = dalloc(c, &c->d_err, 1);
 	if (!r)
+		r = dalloc(c, &c->d_count, 2);
+	if (!r)
 		r = tdv_check(c, hipMemset(c->d_err, 0, sizeof(unsigned)), "memset");
 	if (!r)
 		r = alloc_top(c);
@@ -269,6 +271,7 @@ void tdv_destroy(tdv_context *c)
 	dfree(c->node_up);
 	dfree(c->ewald);
 	dfree(c->d_err);
+	dfree(c->d_count);
 	for (int e = 0; e < EV_N; e++)
 		if (c->ev[e])
 			(void)hipEventDestroy(c->ev[e]);
@@ -489,6 +492,8 @@ int tdv_tree_acceleration(tdv_context *c, int use_bh)
 {
 	if (!c || !c->have_tree)
 		return tdv_fail(c, TDV_ERR_ARG, "walk before a tree build");
+	TDV_HIP(c, hipMemsetAsync(c->d_count, 0, 2 * sizeof(unsigned long long),
+			c->stream));
 	(void)hipEventRecord(c->ev[EV_WALK0], c->stream);
 	TDV_TRY(tdv_walk64(c, use_bh));
 	(void)hipEventRecord(c->ev[EV_WALK1], c->stream);
@@ -498,6 +503,8 @@ int tdv_tree_acceleration(tdv_context *c, int use_bh)
 	TDV_HIP(c, hipStreamSynchronize(c->stream));
 	(void)hipEventElapsedTime(&c->stage_ms[5], c->ev[EV_WALK0], c->ev[EV_WALK1]);
 	(void)hipEventElapsedTime(&c->stage_ms[6], c->ev[EV_WALK1], c->ev[EV_EWALD1]);
+	TDV_HIP(c, hipMemcpy(c->count, c->d_count, sizeof(c->count),
+			hipMemcpyDeviceToHost));
 	return tdv_check_err_flags(c);
 }
 
@@ -749,6 +756,15 @@ int tdv_get_tree(tdv_context *c, void *out, size_t max, size_t *nnodes)
 		r.CoM[2] = a[j].z;
 		rec[j] = r;
 	}
+	return 0;
+}
+
+int tdv_interaction_counts(tdv_context *c, uint64_t counts[2])
+{
+	if (!c || !counts)
+		return TDV_ERR_ARG;
+	counts[0] = c->count[0];
+	counts[1] = c->count[1];
 	return 0;
 }
 

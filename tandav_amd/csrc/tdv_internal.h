@@ -93,6 +93,8 @@ struct tdv_context {
 	int have_ewald;
 
 	unsigned *d_err;
+	unsigned long long *d_count;     /* [2] interactions: tree, Ewald */
+	unsigned long long count[2];
 	int have_domain, have_tree;
 
 	hipEvent_t ev[16];

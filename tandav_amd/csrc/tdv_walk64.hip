@@ -61,6 +61,17 @@ struct Acc {
 	double x, y, z, cost;
 };
 
+/* sum of P.Cost over the launch: one atomic per wavefront */
+__device__ __forceinline__ void count_interactions(double cost,
+		unsigned long long *counter)
+{
+	unsigned long long v = (unsigned long long)cost;
+	for (int d = 32; d >= 1; d >>= 1)
+		v += __shfl_xor(v, d, 64);
+	if ((threadIdx.x & 63) == 0 && v)
+		atomicAdd(counter, v);
+}
+
 /* interact (Gravity/tree_accel.c:362-394): Dehnen K1 kernel, h < 0 */
 __device__ __forceinline__ void interact(Acc &a, double m, double dx,
 		double dy, double dz, double r2, const WalkParams &w)
@@ -100,19 +111,18 @@ __global__ void __launch_bounds__(256) k_walk64(int nact,
 		double *__restrict__ ay, double *__restrict__ az,
 		double *__restrict__ gx, double *__restrict__ gy,
 		double *__restrict__ gz, double *__restrict__ cost, int npart_total,
-		unsigned *err)
+		unsigned *err, unsigned long long *counter)
 {
 	__shared__ TopShared S;
 	load_top(S, T, w.ntop);
 	const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-	if (gid >= nact)
-		return;
-	const int ip = active[gid];
+	const bool live = gid < nact;
+	const int ip = live ? active[gid] : 0;
 	const double4 p = pm[ip];
 	const double fac = last_acc[ip] / w.G * w.alpha_rel;
 	Acc a = { 0, 0, 0, 0 };
 
-	for (int j = 0; j < w.ntop; j++) {
+	for (int j = 0; live && j < w.ntop; j++) {
 		/* interact_with_topnode, Gravity/tree_accel.c:137-176 */
 		const double nSize = w.dom_size / ((double)(1ull << S.level[j]));
 		{
@@ -199,13 +209,16 @@ __global__ void __launch_bounds__(256) k_walk64(int nact,
 		}
 	}
 	/* add_recv_to, Gravity/tree_accel.c:110-129 */
-	ax[ip] = ax[ip] + a.x;
-	ay[ip] = ay[ip] + a.y;
-	az[ip] = az[ip] + a.z;
-	gx[ip] = a.x;
-	gy[ip] = a.y;
-	gz[ip] = a.z;
-	cost[ip] = a.cost;
+	if (live) {
+		ax[ip] = ax[ip] + a.x;
+		ay[ip] = ay[ip] + a.y;
+		az[ip] = az[ip] + a.z;
+		gx[ip] = a.x;
+		gy[ip] = a.y;
+		gz[ip] = a.z;
+		cost[ip] = a.cost;
+	}
+	count_interactions(a.cost, counter);
 }
 
 /* Ewald_Correction, Gravity/periodic.c:35-113 (table already / Box^2) */
@@ -283,19 +296,19 @@ __global__ void __launch_bounds__(256) k_walk64_ewald(int nact,
 		WalkParams w, double *__restrict__ ax, double *__restrict__ ay,
 		double *__restrict__ az, double *__restrict__ gx,
 		double *__restrict__ gy, double *__restrict__ gz,
-		double *__restrict__ cost, int npart_total, unsigned *err)
+		double *__restrict__ cost, int npart_total, unsigned *err,
+		unsigned long long *counter)
 {
 	__shared__ TopShared S;
 	load_top(S, T, w.ntop);
 	const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-	if (gid >= nact)
-		return;
-	const int ip = active[gid];
+	const bool live = gid < nact;
+	const int ip = live ? active[gid] : 0;
 	const double4 p = pm[ip];
 	const double fac = last_acc[ip] / w.G * w.alpha_rel;
 	Acc a = { 0, 0, 0, 0 };
 
-	for (int j = 0; j < w.ntop; j++) {
+	for (int j = 0; live && j < w.ntop; j++) {
 		/* interact_with_topnode, Gravity/tree_periodic.c:112-177 */
 		const double s = w.dom_size / (double)(1ull << S.level[j]);
 		bool want_open = false;
@@ -387,13 +400,16 @@ __global__ void __launch_bounds__(256) k_walk64_ewald(int nact,
 		}
 	}
 	/* add_recv_to, Gravity/tree_periodic.c:91-110 */
-	ax[ip] = ax[ip] + a.x;
-	ay[ip] = ay[ip] + a.y;
-	az[ip] = az[ip] + a.z;
-	gx[ip] = gx[ip] + a.x;
-	gy[ip] = gy[ip] + a.y;
-	gz[ip] = gz[ip] + a.z;
-	cost[ip] = cost[ip] + a.cost;
+	if (live) {
+		ax[ip] = ax[ip] + a.x;
+		ay[ip] = ay[ip] + a.y;
+		az[ip] = az[ip] + a.z;
+		gx[ip] = gx[ip] + a.x;
+		gy[ip] = gy[ip] + a.y;
+		gz[ip] = gz[ip] + a.z;
+		cost[ip] = cost[ip] + a.cost;
+	}
+	count_interactions(a.cost, counter);
 }
 
 static WalkParams walk_params(const tdv_context *c, int use_bh)
@@ -426,7 +442,7 @@ int tdv_walk64(tdv_context *c, int use_bh)
 			dim3(256), 0, c->stream, c->nactive, c->active, c->pm, c->last_acc,
 			c->top, c->node_a, c->node_b, c->node_c, w, c->acc[0], c->acc[1],
 			c->acc[2], c->gacc[0], c->gacc[1], c->gacc[2], c->cost, (int)c->n,
-			c->d_err);
+			c->d_err, c->d_count);
 	return tdv_check(c, hipGetLastError(), "k_walk64");
 }
 
@@ -441,7 +457,8 @@ int tdv_walk64_ewald(tdv_context *c, int use_bh)
 			dim3(tdv_blocks((size_t)c->nactive, 256)), dim3(256), 0, c->stream,
 			c->nactive, c->active, c->pm, c->last_acc, c->top, c->node_a,
 			c->node_b, c->node_c, c->ewald, w, c->acc[0], c->acc[1], c->acc[2],
-			c->gacc[0], c->gacc[1], c->gacc[2], c->cost, (int)c->n, c->d_err);
+			c->gacc[0], c->gacc[1], c->gacc[2], c->cost, (int)c->n, c->d_err,
+			c->d_count + 1);
 	return tdv_check(c, hipGetLastError(), "k_walk64_ewald");
 }
 

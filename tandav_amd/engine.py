@@ -82,6 +82,8 @@ def load_library(path=LIB_PATH):
                                     ctypes.POINTER(ctypes.c_int)]
     L.tdv_get_tree.argtypes = [_ctx, _vp, _sz, ctypes.POINTER(_sz)]
     L.tdv_stage_times.argtypes = [_ctx, ctypes.POINTER(ctypes.c_float)]
+    L.tdv_interaction_counts.argtypes = [_ctx,
+                                         ctypes.POINTER(ctypes.c_uint64)]
     L.tdv_num_particles.argtypes = [_ctx]
     L.tdv_num_particles.restype = _sz
     _lib = L
@@ -301,6 +303,13 @@ class Engine:
                                        n.value, ctypes.byref(n)),
                  "tdv_get_tree")
         return buf
+
+    def interaction_counts(self):
+        """(tree-walk interactions, Ewald interactions) of the last walk."""
+        c = (ctypes.c_uint64 * 2)()
+        self._ok(self.lib.tdv_interaction_counts(self._c, c),
+                 "tdv_interaction_counts")
+        return int(c[0]), int(c[1])
 
     def stage_times(self):
         ms = (ctypes.c_float * 8)()
