@@ -39,6 +39,10 @@ WORKLOADS = {
 }
 
 
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
 def make_workload(name):
     from tandav_amd import ics
     w = WORKLOADS[name]
@@ -54,10 +58,12 @@ def run_gpu(pos, m, soft, w, steps, warmup):
     e.set_particles(pos, m)
     e.setup_domain()
     e.domain_decomposition()
+    log("prepare step (BH walk, then relative walk)")
     e.compute_acceleration(prepare_step=True)       # BH walk, then relative
-    for _ in range(warmup):
+    for i in range(warmup):
         e.domain_decomposition()
         e.compute_acceleration(prepare_step=False)
+        log("warmup step %d: %s" % (i, e.stage_times()))
     stage = {k: 0.0 for k in e.stage_times()}
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -65,6 +71,7 @@ def run_gpu(pos, m, soft, w, steps, warmup):
         e.compute_acceleration(prepare_step=False)
         for k, v in e.stage_times().items():
             stage[k] += v
+        log("timed step done")
     t1 = time.perf_counter()
     cost = e.get("cost")["cost"]
     return e, (t1 - t0) / steps, {k: v / steps for k, v in stage.items()}, cost
@@ -83,8 +90,10 @@ def cpu_baseline(pos, m, soft, w, sample, nthreads):
     t0 = time.perf_counter()
     s.domain_decomposition()
     t1 = time.perf_counter()
+    log("cpu: decomposition %.2fs" % (t1 - t0))
     s.tree_build()
     t2 = time.perf_counter()
+    log("cpu: tree build %.2fs" % (t2 - t1))
     g = np.random.Generator(np.random.Philox(99))
     pick = g.choice(n, sample, replace=False)
     s.time_bin[:] = 1
@@ -95,9 +104,11 @@ def cpu_baseline(pos, m, soft, w, sample, nthreads):
     s.last_acc_mag[:] = np.linalg.norm(s.accelerations(), axis=1)
     for d in range(3):
         s.acc[d][:] = 0
+    log("cpu: BH pass on the sample done")
     t3 = time.perf_counter()
     s.tree_acceleration(use_bh=False)
     t4 = time.perf_counter()
+    log("cpu: relative+Ewald walk on %d particles %.2fs" % (sample, t4 - t3))
     t_step = (t1 - t0) + (t2 - t1) + (t4 - t3) * n / sample
     return {"value": n / t_step, "unit": "force-evals/s", "cores": nthreads,
             "kind": "port",
@@ -123,8 +134,10 @@ def main():
               % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    log("generating %s" % args.workload)
     pos, vel, m, soft, w = make_workload(args.workload)
     n = len(m)
+    log("N = %d" % n)
     e, t_step, stage, cost = run_gpu(pos, m, soft, w, args.steps, args.warmup)
     ms_step = t_step * 1e3
 
