@@ -77,9 +77,12 @@ def run_gpu(pos, m, soft, w, steps, warmup):
     return e, (t1 - t0) / steps, {k: v / steps for k, v in stage.items()}, cost
 
 
-def cpu_baseline(pos, m, soft, w, sample, nthreads):
+def cpu_baseline(pos, m, soft, w, sample, nthreads, gpu_acc, gpu_cost):
     """Oracle on the host cores: decomposition + build on every particle,
-    the BH-free steady-state walks (relative + Ewald) on a random sample."""
+    then the steady-state walks (relative + Ewald) on a random sample of
+    particles with the same previous-step |a| the GPU step used, so the
+    sample does exactly the GPU's work (its Cost is checked against the
+    GPU's) and scales to the whole step."""
     import oracle
     n = len(m)
     table = oracle.cached_ewald_table()
@@ -95,28 +98,25 @@ def cpu_baseline(pos, m, soft, w, sample, nthreads):
     t2 = time.perf_counter()
     log("cpu: tree build %.2fs" % (t2 - t1))
     g = np.random.Generator(np.random.Philox(99))
-    pick = g.choice(n, sample, replace=False)
+    pick = np.sort(g.choice(n, sample, replace=False))
     s.time_bin[:] = 1
     s.time_bin[pick] = 0
     s.set_active_bin(0)
-    # relative criterion needs |a| of the previous step: one BH pass first
-    s.tree_acceleration(use_bh=True)
-    s.last_acc_mag[:] = np.linalg.norm(s.accelerations(), axis=1)
-    for d in range(3):
-        s.acc[d][:] = 0
-    log("cpu: BH pass on the sample done")
+    s.last_acc_mag[:] = gpu_acc          # same PH order as the GPU arrays
     t3 = time.perf_counter()
     s.tree_acceleration(use_bh=False)
     t4 = time.perf_counter()
     log("cpu: relative+Ewald walk on %d particles %.2fs" % (sample, t4 - t3))
+    same_cost = bool(np.array_equal(s.cost[pick], gpu_cost[pick]))
     t_step = (t1 - t0) + (t2 - t1) + (t4 - t3) * n / sample
     return {"value": n / t_step, "unit": "force-evals/s", "cores": nthreads,
             "kind": "port",
             "sample": "oracle (C99+OpenMP restatement of the reference path): "
                       "decomposition %.2fs + tree build %.2fs on all %d "
-                      "particles, relative+Ewald walk on %d random particles "
-                      "%.2fs scaled by N/sample" % (t1 - t0, t2 - t1, n,
-                                                     sample, t4 - t3)}
+                      "particles, steady-state relative+Ewald walk on %d "
+                      "random particles %.2fs scaled by N/sample; sample "
+                      "Cost identical to the GPU's: %s"
+                      % (t1 - t0, t2 - t1, n, sample, t4 - t3, same_cost)}
 
 
 def main():
@@ -125,7 +125,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-sample", type=int, default=4096)
+    ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -139,6 +139,7 @@ def main():
     n = len(m)
     log("N = %d" % n)
     e, t_step, stage, cost = run_gpu(pos, m, soft, w, args.steps, args.warmup)
+    last_cost = cost.copy()
     ms_step = t_step * 1e3
 
     # dominant kernel and its roofline (HIP events on the engine's stream)
@@ -163,8 +164,12 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
+        # the |a| the GPU's last step opened nodes with (Last_Acc_Mag,
+        # accel.c:28-43) and the Cost that step produced
+        g = e.get("last_acc_mag")
         cpu = cpu_baseline(pos, m, soft, w, min(args.cpu_sample, n),
-                           nthreads=min(os.cpu_count() or 1, 16))
+                           min(os.cpu_count() or 1, 16),
+                           g["last_acc_mag"], last_cost)
 
     walk_s = (walk_ms + ewald_ms) * 1e-3
     line = {
