@@ -1,20 +1,25 @@
-"""Force-path benchmark on one MI355X.
+"""Force-path benchmark on one MI355X: FP32 group walk vs FP64 walk.
 
 Workload (default ``c2``): periodic unit box, 128^3 = 2,097,152 particles on a
 grid displaced by the Zel'dovich field of P(k) ~ k^-2.5 (rms |psi| = 0.1 grid
 spacing, Philox seed 7), Plummer softening 1/40 grid spacing, relative
-opening criterion alpha = 0.005, 65^3 FP64 Ewald table.  A step is one
-domain decomposition (keys, sort, reorder, top nodes) plus
-Compute_Acceleration on a regular step (tree build, relative FP64 walk,
-Ewald walk), all on device-resident data.
+opening criterion alpha = 0.005, 65^3 Ewald table.  A step is one domain
+decomposition (PH keys, radix sort, reorder, top nodes) plus
+Compute_Acceleration on a regular step (octree build, relative walk with
+the Ewald correction), all on device-resident data.  The step runs twice:
+with the FP64 walk (the drop-in path: the reference's interaction sets) and
+with the FP32 group walk.
 
 Output: one JSON line.  ``value`` = force evaluations per second of the
-whole step (N_active / t_step).  ``roofline`` covers the dominant kernel,
-timed with HIP events on the engine's stream, counting 20 flops per
-monopole interaction (and 60 per Ewald interaction).  ``cpu_baseline``
-times the CPU oracle (the C/OpenMP restatement of the reference path) on
-the host cores: decomposition and build on all particles, the walks on a
-bounded random sample of active particles, scaled to the same unit.
+whole step with the FP32 group walk (N_active / t_step); ``fp64`` holds the
+same for the FP64 walk.  Accuracy: |da|/|a| of both walks against the FP64
+direct summation with the Ewald correction of every pair
+(Gravity_Forcetest's role, on the GPU) on sampled particles, median and
+p99.  ``roofline`` covers the dominant kernel, timed with HIP events on the
+engine's stream, counting 20 flops per monopole interaction and 60 per
+Ewald correction.  ``cpu_baseline`` times the CPU oracle (the C/OpenMP
+restatement of the reference path) on the host cores: decomposition and
+build on all particles, the FP64 walks on a bounded sample of particles.
 """
 import argparse
 import json
@@ -33,9 +38,10 @@ FLOPS_PER_EWALD = 60.0
 
 WORKLOADS = {
     "c2": dict(ngrid=128, seed=7, alpha=0.005, desc="periodic Zel'dovich box "
-               "128^3, FP64 walk + Ewald (drop-in path)"),
+               "128^3 (2,097,152 particles), relative criterion 0.005, "
+               "Ewald-corrected"),
     "c2_small": dict(ngrid=64, seed=7, alpha=0.005, desc="periodic "
-                     "Zel'dovich box 64^3, FP64 walk + Ewald"),
+                     "Zel'dovich box 64^3, relative criterion 0.005"),
 }
 
 
@@ -51,19 +57,20 @@ def make_workload(name):
     return pos, vel, m, soft, w
 
 
-def run_gpu(pos, m, soft, w, steps, warmup):
+def run_gpu(pos, m, soft, w, steps, warmup, precision, ewald_size):
     from tandav_amd import Engine
     e = Engine(periodic=True, boxsize=1.0, softening=soft, theta_bh=0.1,
-               alpha_rel=w["alpha"], compat=True)
+               alpha_rel=w["alpha"], compat=True, precision=precision,
+               ewald_size=ewald_size)
     e.set_particles(pos, m)
     e.setup_domain()
     e.domain_decomposition()
-    log("prepare step (BH walk, then relative walk)")
-    e.compute_acceleration(prepare_step=True)       # BH walk, then relative
+    log("%s: prepare step (BH walk, then relative walk)" % precision)
+    e.compute_acceleration(prepare_step=True)
     for i in range(warmup):
         e.domain_decomposition()
         e.compute_acceleration(prepare_step=False)
-        log("warmup step %d: %s" % (i, e.stage_times()))
+        log("%s: warmup step %d" % (precision, i))
     stage = {k: 0.0 for k in e.stage_times()}
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -71,13 +78,18 @@ def run_gpu(pos, m, soft, w, steps, warmup):
         e.compute_acceleration(prepare_step=False)
         for k, v in e.stage_times().items():
             stage[k] += v
-        log("timed step done")
+        log("%s: timed step done" % precision)
     t1 = time.perf_counter()
-    cost = e.get("cost")["cost"]
-    return e, (t1 - t0) / steps, {k: v / steps for k, v in stage.items()}, cost
+    return e, (t1 - t0) / steps, {k: v / steps for k, v in stage.items()}
 
 
-def cpu_baseline(pos, m, soft, w, sample, nthreads, gpu_acc, gpu_cost):
+def errors(a, ref):
+    r = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    return {"median": float(np.median(r)), "p99": float(np.percentile(r, 99)),
+            "max": float(r.max())}
+
+
+def cpu_baseline(pos, m, soft, w, sample, nthreads, gpu_last, gpu_cost):
     """Oracle on the host cores: decomposition + build on every particle,
     then the steady-state walks (relative + Ewald) on a random sample of
     particles with the same previous-step |a| the GPU step used, so the
@@ -102,7 +114,7 @@ def cpu_baseline(pos, m, soft, w, sample, nthreads, gpu_acc, gpu_cost):
     s.time_bin[:] = 1
     s.time_bin[pick] = 0
     s.set_active_bin(0)
-    s.last_acc_mag[:] = gpu_acc          # same PH order as the GPU arrays
+    s.last_acc_mag[:] = gpu_last          # same PH order as the GPU arrays
     t3 = time.perf_counter()
     s.tree_acceleration(use_bh=False)
     t4 = time.perf_counter()
@@ -111,26 +123,29 @@ def cpu_baseline(pos, m, soft, w, sample, nthreads, gpu_acc, gpu_cost):
     t_step = (t1 - t0) + (t2 - t1) + (t4 - t3) * n / sample
     return {"value": n / t_step, "unit": "force-evals/s", "cores": nthreads,
             "kind": "port",
-            "sample": "oracle (C99+OpenMP restatement of the reference path): "
-                      "decomposition %.2fs + tree build %.2fs on all %d "
-                      "particles, steady-state relative+Ewald walk on %d "
-                      "random particles %.2fs scaled by N/sample; sample "
-                      "Cost identical to the GPU's: %s"
+            "sample": "oracle (C99+OpenMP restatement of the reference path, "
+                      "FP64 walk): decomposition %.2fs + tree build %.2fs on "
+                      "all %d particles, steady-state relative+Ewald walk on "
+                      "%d random particles %.2fs scaled by N/sample; sample "
+                      "Cost identical to the GPU FP64 walk's: %s"
                       % (t1 - t0, t2 - t1, n, sample, t4 - t3, same_cost)}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-sample", type=int, default=512)
+    ap.add_argument("--ewald-size", type=float, default=1.0 / 64)
+    ap.add_argument("--error-sample", type=int, default=8192)
+    ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["fp32", "fp64"], default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus %d is not "
-              "supported (the multi-GPU sharding is a design note only)"
+              "supported (multi-GPU sharding is a design note only)"
               % args.gpus, file=sys.stderr)
         sys.exit(2)
 
@@ -138,59 +153,91 @@ def main():
     pos, vel, m, soft, w = make_workload(args.workload)
     n = len(m)
     log("N = %d" % n)
-    e, t_step, stage, cost = run_gpu(pos, m, soft, w, args.steps, args.warmup)
-    last_cost = cost.copy()
-    ms_step = t_step * 1e3
+    g = np.random.Generator(np.random.Philox(5))
+    idx = np.sort(g.choice(n, min(n, args.error_sample), replace=False))
 
-    # dominant kernel and its roofline (HIP events on the engine's stream)
-    walk_ms, ewald_ms = stage["walk"], stage["ewald"]
-    tree_inter, ewald_inter = e.interaction_counts()
-    if walk_ms >= ewald_ms:
-        kern, t_k = "k_walk64", walk_ms
-        flops = tree_inter * FLOPS_PER_INTERACTION
+    res = {}
+    ref = None
+    for prec in ("fp64", "fp32"):
+        if args.only and prec != args.only:
+            continue
+        e, t_step, stage = run_gpu(pos, m, soft, w, args.steps, args.warmup,
+                                   prec, args.ewald_size)
+        got = e.get("acc", "cost", "last_acc_mag")
+        if ref is None:
+            t0 = time.perf_counter()
+            ref = e.forcetest(idx)       # particle order is the same PH order
+            log("direct-sum oracle on %d particles %.2fs"
+                % (len(idx), time.perf_counter() - t0))
+        newton, ewald = e.interaction_counts()
+        res[prec] = dict(e=e, t_step=t_step, stage=stage, got=got,
+                         err=errors(got["acc"][idx], ref), newton=newton,
+                         ewald=ewald)
+        log("%s: %.1f ms/step, err %s" % (prec, t_step * 1e3,
+                                           res[prec]["err"]))
+
+    head = res["fp32"] if "fp32" in res else res["fp64"]
+    if "fp32" in res:
+        kern = "k_walk32"
+        t_k = head["stage"]["walk"]
+        flops = (head["newton"] * FLOPS_PER_INTERACTION
+                 + head["ewald"] * FLOPS_PER_EWALD)
+        peak, dtype = PEAK["f32"], "f32"
     else:
-        kern, t_k = "k_walk64_ewald", ewald_ms
-        flops = ewald_inter * FLOPS_PER_EWALD
+        walk_ms, ewald_ms = head["stage"]["walk"], head["stage"]["ewald"]
+        if walk_ms >= ewald_ms:
+            kern, t_k = "k_walk64", walk_ms
+            flops = head["newton"] * FLOPS_PER_INTERACTION
+        else:
+            kern, t_k = "k_walk64_ewald", ewald_ms
+            flops = head["ewald"] * FLOPS_PER_EWALD
+        peak, dtype = PEAK["f64"], "f64"
     achieved = flops / (t_k * 1e-3)
-    roof = {"bound": "fp64-valu", "kernel": kern,
-            "achieved": achieved / 1e12, "peak": PEAK["f64"] / 1e12,
-            "unit": "TFLOP/s", "frac": achieved / PEAK["f64"],
-            "traffic": None,
-            "flops_per_interaction": FLOPS_PER_INTERACTION
-            if kern == "k_walk64" else FLOPS_PER_EWALD,
-            "interactions_per_launch": tree_inter if kern == "k_walk64"
-            else ewald_inter,
-            "avg_launch_ms": t_k}
+    roof = {"bound": "valu", "kernel": kern, "achieved": achieved / 1e12,
+            "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
+            "traffic": None, "avg_launch_ms": t_k,
+            "flops_per_launch": flops,
+            "counting": "20 flops per monopole interaction, 60 per Ewald "
+                        "correction; interactions counted on the device"}
 
     cpu = None
-    if not args.no_cpu_baseline:
-        # the |a| the GPU's last step opened nodes with (Last_Acc_Mag,
-        # accel.c:28-43) and the Cost that step produced
-        g = e.get("last_acc_mag")
+    if not args.no_cpu_baseline and "fp64" in res:
+        r64 = res["fp64"]
         cpu = cpu_baseline(pos, m, soft, w, min(args.cpu_sample, n),
                            min(os.cpu_count() or 1, 16),
-                           g["last_acc_mag"], last_cost)
+                           r64["got"]["last_acc_mag"], r64["got"]["cost"])
 
-    walk_s = (walk_ms + ewald_ms) * 1e-3
+    def summary(r):
+        walk_s = (r["stage"]["walk"] + r["stage"]["ewald"]) * 1e-3
+        return {"force_evals_per_s": n / r["t_step"],
+                "ms_per_step": r["t_step"] * 1e3,
+                "walk_force_evals_per_s": n / walk_s,
+                "interactions_per_s": (r["newton"] + r["ewald"]) / walk_s,
+                "newton_interactions": r["newton"],
+                "ewald_interactions": r["ewald"],
+                "stages_ms": {k: round(v, 4) for k, v in r["stage"].items()},
+                "err_vs_direct_ewald": r["err"]}
+
     line = {
         "metric": "tree-force evals/s per step, FP32 vs FP64 walk, at p99 "
                   "|da|/|a| <= 2e-3",
-        "value": n / t_step,
+        "value": n / head["t_step"],
         "unit": "force-evals/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": head["t_step"] * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
+        "dtype": dtype, "data": "synthetic",
         "config": {"workload": w["desc"], "n_particles": n,
                    "softening": soft, "alpha_rel": w["alpha"],
-                   "walk": "fp64", "parallelism": "single GPU"},
-        "stages_ms": {k: round(v, 4) for k, v in stage.items()},
-        "walk_force_evals_per_s": n / walk_s,
-        "interactions_per_s": float(cost.sum()) / walk_s,
-        "interactions_per_particle": float(cost.mean()),
+                   "ewald_size": args.ewald_size,
+                   "error_sample": len(idx), "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    for prec, r in res.items():
+        line[prec] = summary(r)
+    if "fp32" in res and "fp64" in res:
+        line["fp32_over_fp64"] = res["fp64"]["t_step"] / res["fp32"]["t_step"]
     print(json.dumps(line), flush=True)
 
 

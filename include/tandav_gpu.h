@@ -52,7 +52,14 @@ typedef struct {
 	                          (Gravity/tree_accel.c:183-203); 0: their own
 	                          particles                                        */
 	int device;            /* HIP device ordinal                               */
+	int precision;         /* TDV_WALK_FP64: the reference's per-particle walk
+	                          (identical interaction sets); TDV_WALK_FP32: the
+	                          group walk on fixed-point separations           */
+	double ewald_size;     /* FP32 walk: largest node (box units) whose Ewald
+	                          correction is taken at its centre of mass       */
 } tdv_params;
+
+enum { TDV_WALK_FP64 = 0, TDV_WALK_FP32 = 1 };
 
 /* Setup_Gravity_Tree (Gravity/tree_build.c:124-142) and, for periodic runs,
  * Gravity_Periodic_Init (Gravity/periodic.c:180-233): softening lengths,
@@ -102,6 +109,12 @@ int tdv_tree_acceleration(tdv_context *ctx, int use_bh);
  * decomposition happened, BH walk + relative walk on the prepare step,
  * relative walk otherwise. */
 int tdv_compute_acceleration(tdv_context *ctx, int prepare_step);
+
+/* Gravity_Forcetest (Gravity/forcetest.c:24-174): FP64 direct summation
+ * over all particles (plus the tabulated Ewald correction of every pair on
+ * periodic runs) for nt targets given by index in the current particle
+ * order; acc_out = 3*nt doubles. */
+int tdv_forcetest(tdv_context *ctx, int nt, const int *idx, double *acc_out);
 
 /* Element-wise batched forms of the pure functions (host arrays). */
 /* Peano_Key / Reversed_Peano_Key (peano.c:122-276); keys_out = 2n words. */

@@ -224,6 +224,8 @@ int tdv_create(const tdv_params *p, tdv_context **out)
 		return TDV_ERR_ARG;
 	if (p->periodic && !(p->boxsize > 0))
 		return TDV_ERR_ARG;
+	if (p->precision != TDV_WALK_FP64 && p->precision != TDV_WALK_FP32)
+		return TDV_ERR_ARG;
 	tdv_context *c = new (std::nothrow) tdv_context();
 	if (!c)
 		return TDV_ERR_NOMEM;
@@ -270,6 +272,10 @@ void tdv_destroy(tdv_context *c)
 	dfree(c->node_c);
 	dfree(c->node_up);
 	dfree(c->ewald);
+	dfree(c->ewald32);
+	dfree(c->p32);
+	dfree(c->a32);
+	dfree(c->b32);
 	dfree(c->d_err);
 	dfree(c->d_count);
 	for (int e = 0; e < EV_N; e++)
@@ -495,10 +501,15 @@ int tdv_tree_acceleration(tdv_context *c, int use_bh)
 	TDV_HIP(c, hipMemsetAsync(c->d_count, 0, 2 * sizeof(unsigned long long),
 			c->stream));
 	(void)hipEventRecord(c->ev[EV_WALK0], c->stream);
-	TDV_TRY(tdv_walk64(c, use_bh));
-	(void)hipEventRecord(c->ev[EV_WALK1], c->stream);
-	if (c->par.periodic)
-		TDV_TRY(tdv_walk64_ewald(c, use_bh));
+	if (c->par.precision == TDV_WALK_FP32) {
+		TDV_TRY(tdv_walk32(c, use_bh));     /* Newton + Ewald in one pass */
+		(void)hipEventRecord(c->ev[EV_WALK1], c->stream);
+	} else {
+		TDV_TRY(tdv_walk64(c, use_bh));
+		(void)hipEventRecord(c->ev[EV_WALK1], c->stream);
+		if (c->par.periodic)
+			TDV_TRY(tdv_walk64_ewald(c, use_bh));
+	}
 	(void)hipEventRecord(c->ev[EV_EWALD1], c->stream);
 	TDV_HIP(c, hipStreamSynchronize(c->stream));
 	(void)hipEventElapsedTime(&c->stage_ms[5], c->ev[EV_WALK0], c->ev[EV_WALK1]);
@@ -520,6 +531,35 @@ int tdv_compute_acceleration(tdv_context *c, int prepare_step)
 		tdv_launch_safe_last_accel(c);
 	}
 	return tdv_tree_acceleration(c, 0);
+}
+
+int tdv_forcetest(tdv_context *c, int nt, const int *idx, double *acc_out)
+{
+	if (!c || nt < 0 || (nt && (!idx || !acc_out)) || !c->n)
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_forcetest: bad arguments");
+	if (!nt)
+		return 0;
+	for (int t = 0; t < nt; t++)
+		if (idx[t] < 0 || (size_t)idx[t] >= c->n)
+			return tdv_fail(c, TDV_ERR_ARG, "tdv_forcetest: index out of range");
+	int *di = nullptr;
+	double *dout = nullptr;
+	int r = dalloc(c, &di, (size_t)nt);
+	if (!r)
+		r = dalloc(c, &dout, 3 * (size_t)nt);
+	if (!r)
+		r = tdv_check(c, hipMemcpyAsync(di, idx, nt * sizeof(int),
+				hipMemcpyHostToDevice, c->stream), "copy");
+	if (!r)
+		r = tdv_direct_sum(c, nt, di, dout);
+	if (!r)
+		r = tdv_check(c, hipMemcpyAsync(acc_out, dout, 3 * nt * sizeof(double),
+				hipMemcpyDeviceToHost, c->stream), "copy");
+	if (!r)
+		r = tdv_check(c, hipStreamSynchronize(c->stream), "forcetest");
+	dfree(di);
+	dfree(dout);
+	return r;
 }
 
 int tdv_peano_keys(tdv_context *c, size_t n, const double *x, const double *y,

@@ -92,6 +92,11 @@ struct tdv_context {
 	double *ewald;                   /* 4 * TDV_NE3, normalised */
 	int have_ewald;
 
+	/* FP32 group walk: fixed-point records */
+	int4 *p32, *a32, *b32;
+	size_t n32_cap;
+	float *ewald32;
+
 	unsigned *d_err;
 	unsigned long long *d_count;     /* [2] interactions: tree, Ewald */
 	unsigned long long count[2];
@@ -127,6 +132,8 @@ int tdv_build_tree(tdv_context *c);
 /* ---- launchers (tdv_walk64.hip, tdv_ewald.hip) ---- */
 int tdv_walk64(tdv_context *c, int use_bh);
 int tdv_walk64_ewald(tdv_context *c, int use_bh);
+int tdv_walk32(tdv_context *c, int use_bh);
+int tdv_direct_sum(tdv_context *c, int nt, const int *idx_dev, double *out_dev);
 void tdv_launch_safe_last_accel(tdv_context *c);
 int tdv_ewald_table_device(tdv_context *c, double *out4_unnormalised);
 void tdv_launch_ewald_correction(tdv_context *c, size_t n, const double *dr,

@@ -1,0 +1,412 @@
+/*
+ * FP32 group walk.  One wavefront walks the tree for a PH-contiguous group
+ * of 64 active particles (one per lane) with a single shared cursor: a node
+ * is opened when any lane's criterion (the relative criterion of
+ * Gravity/tree_accel.c:257 or the BH criterion of :339, plus the "inside"
+ * test of :264-281) asks for it, otherwise every lane takes the node's
+ * monopole.  Control flow is uniform across the wavefront, node records are
+ * fetched with scalar loads, and every lane does useful work on every
+ * interaction.
+ *
+ * Separations come from 32-bit fixed-point coordinates (2^32 per box on
+ * periodic runs, so the integer difference is already the nearest image;
+ * 2^31 per domain otherwise): the subtraction is exact and only the
+ * difference is rounded to FP32, so nearby particles do not lose digits to
+ * cancellation.  Masses are pre-scaled so the kernel works in fixed-point
+ * length units.  Lanes accumulate in FP32 and carry the partial sums into
+ * FP64 accumulators every 64 interactions.
+ *
+ * Periodic runs add the Ewald correction in the same traversal: a node that
+ * is taken as a monopole adds Newton + correction at its centre of mass
+ * (the periodic force of a point mass); when a node is opened and is
+ * already small (<= ewald_size) and does not straddle the half box for any
+ * lane, its correction is added once at its centre of mass and its subtree
+ * then adds Newton terms only.  Only the thin layer of bundles that
+ * straddle the half box falls back to per-particle corrections.  The
+ * correction comes from the reference's 65^3 table (Gravity/periodic.c:
+ * 35-113) in FP32, L2-resident.
+ */
+#include "tdv_internal.h"
+#include "tdv_device.h"
+
+struct Walk32Params {
+	int ntop, vs, periodic, use_bh;
+	float theta_bh, alpha_rel;
+	float eps2, inv_eps2, k_inner;   /* fixed units: eps^2, 1/eps^2,
+	                                    1/(16 eps^3) */
+	float ewald_size;                /* correction at nodes <= this size */
+	float half;                      /* half box in fixed units */
+	float box_f;                     /* box in fixed units */
+	float u_scale;                   /* fixed units -> table grid units */
+	int carry_every;
+};
+
+struct Top32 {
+	int4 com[TDV_MAX_TOP];      /* fixed CoM, scaled mass bits */
+	int4 cen[TDV_MAX_TOP];      /* fixed centre, size bits */
+	int target[TDV_MAX_TOP], npart[TDV_MAX_TOP], first[TDV_MAX_TOP];
+};
+
+/* ---- preparation: fixed-point particle / node records -------------------- */
+
+__device__ __forceinline__ unsigned fixp(double x, double o, double S)
+{
+	const double u = (x - o) * S;
+	return (unsigned)(unsigned long long)u;
+}
+
+__global__ void k_prep32_particles(size_t n, const double4 *__restrict__ pm,
+		double o0, double o1, double o2, double S, double mscale,
+		int4 *__restrict__ p32)
+{
+	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const double4 p = pm[i];
+	p32[i] = make_int4((int)fixp(p.x, o0, S), (int)fixp(p.y, o1, S),
+			(int)fixp(p.z, o2, S), __float_as_int((float)(p.w * mscale)));
+}
+
+__global__ void k_prep32_nodes(size_t nn, const double4 *__restrict__ na,
+		const double4 *__restrict__ nb, double o0, double o1, double o2,
+		double S, double mscale, int4 *__restrict__ a32,
+		int4 *__restrict__ b32)
+{
+	const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= nn)
+		return;
+	const double4 a = na[j], b = nb[j];
+	a32[j] = make_int4((int)fixp(a.x, o0, S), (int)fixp(a.y, o1, S),
+			(int)fixp(a.z, o2, S), __float_as_int((float)(a.w * mscale)));
+	b32[j] = make_int4((int)fixp(b.x, o0, S), (int)fixp(b.y, o1, S),
+			(int)fixp(b.z, o2, S), __float_as_int((float)(b.w * S)));
+}
+
+__global__ void k_prep32_ewald(const double *__restrict__ raw,
+		float *__restrict__ t32, double scale)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < 3 * TDV_NE3)
+		t32[i] = (float)(raw[i] * scale);
+}
+
+/* ---- the walk ---------------------------------------------------------- */
+
+struct Acc32 {
+	float x, y, z;
+	double X, Y, Z;
+	int since;
+	unsigned long long cost, ewald;
+};
+
+__device__ __forceinline__ void carry(Acc32 &a)
+{
+	a.X += (double)a.x;
+	a.Y += (double)a.y;
+	a.Z += (double)a.z;
+	a.x = a.y = a.z = 0.f;
+	a.since = 0;
+}
+
+__device__ __forceinline__ void tick(Acc32 &a, int every)
+{
+	if (++a.since >= every)
+		carry(a);
+}
+
+/* K1 kernel (Gravity/tree_accel.c:362-394) in fixed units; m = G m S^2 */
+__device__ __forceinline__ void newton32(Acc32 &a, float m, float dx,
+		float dy, float dz, const Walk32Params &w)
+{
+	const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+	if (r2 == 0.f)
+		return;
+	const float ri = __frsqrt_rn(r2);
+	float fac = m * ri * ri * ri;
+	if (r2 < w.eps2) {
+		const float u2 = r2 * w.inv_eps2;
+		fac = m * (175.f - u2 * (294.f - u2 * 135.f)) * w.k_inner;
+	}
+	a.x = fmaf(fac, dx, a.x);
+	a.y = fmaf(fac, dy, a.y);
+	a.z = fmaf(fac, dz, a.z);
+	a.cost++;
+}
+
+/* Ewald_Correction (Gravity/periodic.c:35-113) on the FP32 table, already
+ * scaled to fixed units and divided by the mass scale */
+__device__ __forceinline__ void ewald32(Acc32 &a, const float *__restrict__ t,
+		float m, float dx, float dy, float dz, const Walk32Params &w)
+{
+	float sx = -1.f, sy = -1.f, sz = -1.f;
+	if (dx < 0.f) {
+		dx = -dx;
+		sx = 1.f;
+	}
+	if (dy < 0.f) {
+		dy = -dy;
+		sy = 1.f;
+	}
+	if (dz < 0.f) {
+		dz = -dz;
+		sz = 1.f;
+	}
+	float u = dx * w.u_scale, v = dy * w.u_scale, q = dz * w.u_scale;
+	int i = min((int)u, TDV_NE - 1), j = min((int)v, TDV_NE - 1),
+			k = min((int)q, TDV_NE - 1);
+	u -= (float)i;
+	v -= (float)j;
+	q -= (float)k;
+	const float w0 = (1.f - u) * (1.f - v), w1 = (1.f - u) * v,
+			w2 = u * (1.f - v), w3 = u * v;
+	const int b = (i * TDV_NE1 + j) * TDV_NE1 + k;
+	const int di = TDV_NE1 * TDV_NE1, dj = TDV_NE1;
+	float f[3];
+#pragma unroll
+	for (int d = 0; d < 3; d++) {
+		const float *F = t + d * TDV_NE3 + b;
+		const float lo = w0 * F[0] + w1 * F[dj] + w2 * F[di] + w3 * F[di + dj];
+		const float hi = w0 * F[1] + w1 * F[dj + 1] + w2 * F[di + 1]
+				+ w3 * F[di + dj + 1];
+		f[d] = fmaf(q, hi - lo, lo);
+	}
+	a.x = fmaf(m * sx, f[0], a.x);
+	a.y = fmaf(m * sy, f[1], a.y);
+	a.z = fmaf(m * sz, f[2], a.z);
+	a.ewald++;
+}
+
+__device__ __forceinline__ float fdiff(int a, int b)
+{
+	return (float)(int)((unsigned)a - (unsigned)b);
+}
+
+/* any lane: particle pos within the node's straddle zone (half box) */
+__device__ __forceinline__ bool straddles(float sx, float sy, float sz,
+		float s, const Walk32Params &w)
+{
+	const float lim = 0.5f * (w.box_f - s);
+	return fabsf(sx) > lim || fabsf(sy) > lim || fabsf(sz) > lim;
+}
+
+__global__ void __launch_bounds__(64) k_walk32(int nact,
+		const int *__restrict__ active, const int4 *__restrict__ p32,
+		const double *__restrict__ last_acc, TopNodes T,
+		const int4 *__restrict__ a32, const int4 *__restrict__ b32,
+		const int4 *__restrict__ nc, const float *__restrict__ tab,
+		Walk32Params w, double *__restrict__ ax, double *__restrict__ ay,
+		double *__restrict__ az, double *__restrict__ gx,
+		double *__restrict__ gy, double *__restrict__ gz,
+		double *__restrict__ cost, unsigned long long *counter, double o0,
+		double o1, double o2, double S, double mscale)
+{
+	__shared__ Top32 TS;
+	for (int t = threadIdx.x; t < w.ntop; t += 64) {
+		const double m = (double)T.mass[t] * mscale;
+		TS.com[t] = make_int4((int)fixp(T.com[3 * t], o0, S),
+				(int)fixp(T.com[3 * t + 1], o1, S),
+				(int)fixp(T.com[3 * t + 2], o2, S), __float_as_int((float)m));
+		TS.target[t] = T.target[t];
+		TS.npart[t] = T.npart[t];
+		TS.first[t] = T.first[t];
+	}
+	__syncthreads();
+
+	const int gid = blockIdx.x * 64 + threadIdx.x;
+	const bool live = gid < nact;
+	const int ip = live ? active[gid] : active[nact - 1];
+	const int4 me = p32[ip];
+	/* relative criterion: open if M s^2 > r^4 |a| alpha / G; with masses
+	 * scaled by G S^2 and lengths in fixed units this is m s^2 > r^4 crit */
+	const float crit = (float)(last_acc[ip] * (double)w.alpha_rel);
+	Acc32 a = { 0.f, 0.f, 0.f, 0.0, 0.0, 0.0, 0, 0ull, 0ull };
+	const int every = w.carry_every;
+
+	for (int j = 0; j < w.ntop; j++) {
+		const int4 tc = TS.com[j];
+		const float tm = __int_as_float(tc.w);
+		const int lvl = TDV_TOP_LEVEL;
+		const float ts = w.box_f / (float)(1 << lvl);
+		const float rx = fdiff(tc.x, me.x), ry = fdiff(tc.y, me.y),
+				rz = fdiff(tc.z, me.z);
+		const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+		bool want = w.use_bh ? (ts * ts > r2 * w.theta_bh)
+				: (tm * ts * ts > (r2 * crit) * r2);
+		/* the top node holds the particle (or is close): walk it */
+		want = want || r2 < 3.f * ts * ts;
+		if (!__any(want)) {
+			newton32(a, tm, rx, ry, rz, w);
+			if (w.periodic)
+				ewald32(a, tab, tm, rx, ry, rz, w);
+			tick(a, every);
+			continue;
+		}
+		if (TS.npart[j] <= w.vs) {
+			const int f = TS.first[j], l = f + TS.npart[j];
+			for (int q = f; q < l; q++) {
+				const int4 pq = p32[q];
+				const float dx = fdiff(pq.x, me.x), dy = fdiff(pq.y, me.y),
+						dz = fdiff(pq.z, me.z);
+				const float m = __int_as_float(pq.w);
+				newton32(a, m, dx, dy, dz, w);
+				if (w.periodic && (dx != 0.f || dy != 0.f || dz != 0.f))
+					ewald32(a, tab, m, dx, dy, dz, w);
+				tick(a, every);
+			}
+			continue;
+		}
+		const int start = TS.target[j];
+		int node = start;
+		int ew_until = w.periodic ? -1 : 0x7FFFFFFF;  /* correction done below */
+		for (;;) {
+			const int4 c = nc[node];
+			if (c.x == 0 && node != start)
+				break;
+			const bool pending = node >= ew_until;
+			if (c.x < 0) {
+				const int f = -c.x - 1, l = f + c.y;
+				bool per_particle = false;
+				if (pending) {
+					/* the bundle's correction at its CoM when it is small
+					 * and straddles the half box for no lane, else per
+					 * particle */
+					const int4 A = a32[node];
+					const int4 B = b32[node];
+					const float s = __int_as_float(B.w);
+					const bool st = straddles(fdiff(B.x, me.x),
+							fdiff(B.y, me.y), fdiff(B.z, me.z), s, w);
+					if (s > w.ewald_size || __any(st)) {
+						per_particle = true;
+					} else {
+						ewald32(a, tab, __int_as_float(A.w), fdiff(A.x, me.x),
+								fdiff(A.y, me.y), fdiff(A.z, me.z), w);
+					}
+				}
+				for (int q = f; q < l; q++) {
+					const int4 pq = p32[q];
+					const float dx = fdiff(pq.x, me.x), dy = fdiff(pq.y, me.y),
+							dz = fdiff(pq.z, me.z);
+					const float m = __int_as_float(pq.w);
+					newton32(a, m, dx, dy, dz, w);
+					if (per_particle && (dx != 0.f || dy != 0.f || dz != 0.f))
+						ewald32(a, tab, m, dx, dy, dz, w);
+					tick(a, every);
+				}
+				node++;
+				continue;
+			}
+			const int4 A = a32[node];
+			const int4 B = b32[node];
+			const float m = __int_as_float(A.w), s = __int_as_float(B.w);
+			const float rx = fdiff(A.x, me.x), ry = fdiff(A.y, me.y),
+					rz = fdiff(A.z, me.z);
+			const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+			bool open;
+			const float sx = fdiff(B.x, me.x), sy = fdiff(B.y, me.y),
+					sz = fdiff(B.z, me.z);
+			if (w.use_bh) {
+				open = s * s > r2 * w.theta_bh;
+			} else {
+				open = m * s * s > (r2 * crit) * r2;
+				open = open || (fabsf(sx) < 0.6f * s && fabsf(sy) < 0.6f * s
+						&& fabsf(sz) < 0.6f * s);
+			}
+			if (!__any(open)) {
+				newton32(a, m, rx, ry, rz, w);
+				if (pending)
+					ewald32(a, tab, m, rx, ry, rz, w);
+				tick(a, every);
+				node += c.x;
+				continue;
+			}
+			if (pending && s <= w.ewald_size
+					&& !__any(straddles(sx, sy, sz, s, w))) {
+				ewald32(a, tab, m, rx, ry, rz, w);
+				ew_until = node + c.x;
+			}
+			node++;
+		}
+	}
+	carry(a);
+	if (live) {
+		ax[ip] = ax[ip] + a.X;
+		ay[ip] = ay[ip] + a.Y;
+		az[ip] = az[ip] + a.Z;
+		gx[ip] = a.X;
+		gy[ip] = a.Y;
+		gz[ip] = a.Z;
+		cost[ip] = (double)(a.cost + a.ewald);
+	}
+	unsigned long long v = live ? a.cost : 0ull, e = live ? a.ewald : 0ull;
+	for (int d = 32; d >= 1; d >>= 1) {
+		v += __shfl_xor(v, d, 64);
+		e += __shfl_xor(e, d, 64);
+	}
+	if (threadIdx.x == 0) {
+		if (v)
+			atomicAdd(counter, v);
+		if (e)
+			atomicAdd(counter + 1, e);
+	}
+}
+
+int tdv_walk32(tdv_context *c, int use_bh)
+{
+	if (!c->nactive)
+		return 0;
+	const size_t n = c->n, nn = c->nnodes;
+	/* fixed-point frame: 2^32 per periodic box, 2^31 per domain otherwise */
+	const double S = c->par.periodic ? 4294967296.0 / c->par.boxsize
+			: 2147483648.0 / c->dom_size;
+	const double o[3] = { c->par.periodic ? 0.0 : c->dom_origin[0],
+		c->par.periodic ? 0.0 : c->dom_origin[1],
+		c->par.periodic ? 0.0 : c->dom_origin[2] };
+	const double mscale = c->par.gravity * S * S;
+	if (!c->p32)
+		TDV_HIP(c, hipMalloc((void **)&c->p32, c->cap * sizeof(int4)));
+	if (c->n32_cap < c->node_cap) {
+		if (c->a32)
+			(void)hipFree(c->a32);
+		if (c->b32)
+			(void)hipFree(c->b32);
+		TDV_HIP(c, hipMalloc((void **)&c->a32, c->node_cap * sizeof(int4)));
+		TDV_HIP(c, hipMalloc((void **)&c->b32, c->node_cap * sizeof(int4)));
+		c->n32_cap = c->node_cap;
+	}
+	hipLaunchKernelGGL(k_prep32_particles, dim3(tdv_blocks(n, 256)), dim3(256),
+			0, c->stream, n, c->pm, o[0], o[1], o[2], S, mscale, c->p32);
+	if (nn)
+		hipLaunchKernelGGL(k_prep32_nodes, dim3(tdv_blocks(nn, 256)),
+				dim3(256), 0, c->stream, nn, c->node_a, c->node_b, o[0], o[1],
+				o[2], S, mscale, c->a32, c->b32);
+	if (c->par.periodic && !c->ewald32) {
+		TDV_HIP(c, hipMalloc((void **)&c->ewald32,
+				3 * (size_t)TDV_NE3 * sizeof(float)));
+		/* table / Box^2 (already in c->ewald), times the inverse mass and
+		 * length scaling: a = (G m S^2) F / (Box^2 S^2) ... / G */
+		hipLaunchKernelGGL(k_prep32_ewald, dim3(tdv_blocks(3 * TDV_NE3, 256)),
+				dim3(256), 0, c->stream, c->ewald, c->ewald32, 1.0 / (S * S));
+	}
+	Walk32Params w;
+	w.ntop = c->ntop;
+	w.vs = c->par.vector_size;
+	w.periodic = c->par.periodic;
+	w.use_bh = use_bh;
+	w.theta_bh = (float)c->par.theta_bh;
+	w.alpha_rel = (float)c->par.alpha_rel;
+	const double ef = c->Epsilon * S;          /* negative, fixed units */
+	w.eps2 = (float)(ef * ef);
+	w.inv_eps2 = (float)(1.0 / (ef * ef));
+	w.k_inner = (float)(1.0 / (16.0 * ef * ef * ef));
+	w.box_f = (float)(c->par.periodic ? 4294967296.0 : 2147483648.0);
+	w.half = 0.5f * w.box_f;
+	w.ewald_size = (float)(c->par.ewald_size * c->par.boxsize * S);
+	w.u_scale = (float)(2.0 * TDV_NE / 4294967296.0);
+	w.carry_every = 64;
+	hipLaunchKernelGGL(k_walk32, dim3(tdv_blocks((size_t)c->nactive, 64)),
+			dim3(64), 0, c->stream, c->nactive, c->active, c->p32, c->last_acc,
+			c->top, c->a32, c->b32, c->node_c, c->ewald32, w, c->acc[0],
+			c->acc[1], c->acc[2], c->gacc[0], c->gacc[1], c->gacc[2], c->cost,
+			c->d_count, o[0], o[1], o[2], S, mscale);
+	return tdv_check(c, hipGetLastError(), "k_walk32");
+}

@@ -33,7 +33,8 @@ class TdvParams(ctypes.Structure):
                 ("gravity", ctypes.c_double), ("softening", ctypes.c_double),
                 ("theta_bh", ctypes.c_double), ("alpha_rel", ctypes.c_double),
                 ("vector_size", ctypes.c_int), ("compat", ctypes.c_int),
-                ("device", ctypes.c_int)]
+                ("device", ctypes.c_int), ("precision", ctypes.c_int),
+                ("ewald_size", ctypes.c_double)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -71,6 +72,7 @@ def load_library(path=LIB_PATH):
         getattr(L, f).argtypes = [_ctx]
     L.tdv_tree_acceleration.argtypes = [_ctx, ctypes.c_int]
     L.tdv_compute_acceleration.argtypes = [_ctx, ctypes.c_int]
+    L.tdv_forcetest.argtypes = [_ctx, ctypes.c_int, _vp, _dp]
     L.tdv_peano_keys.argtypes = [_ctx, _sz, _dp, _dp, _dp, _dp,
                                  ctypes.c_double, ctypes.c_int, _vp]
     L.tdv_sort_keys.argtypes = [_ctx, _sz, _vp, _vp]
@@ -106,12 +108,15 @@ class Engine:
 
     def __init__(self, periodic=False, boxsize=1.0, gravity=1.0,
                  softening=0.01, theta_bh=0.1, alpha_rel=0.02, vector_size=64,
-                 compat=True, device=0, ewald_table=None):
+                 compat=True, device=0, ewald_table=None, precision="fp64",
+                 ewald_size=0.0625):
         self.lib = load_library()
+        prec = {"fp64": 0, "fp32": 1}[precision]
         self.params = TdvParams(int(periodic), float(boxsize), float(gravity),
                                 float(softening), float(theta_bh),
                                 float(alpha_rel), int(vector_size),
-                                int(compat), int(device))
+                                int(compat), int(device), prec,
+                                float(ewald_size))
         self._c = _ctx()
         rc = self.lib.tdv_create(ctypes.byref(self.params),
                                  ctypes.byref(self._c))
@@ -234,6 +239,17 @@ class Engine:
         """Compute_Acceleration (accel.c:15-71)."""
         self._ok(self.lib.tdv_compute_acceleration(self._c, int(prepare_step)),
                  "tdv_compute_acceleration")
+
+    def forcetest(self, idx):
+        """Gravity_Forcetest (Gravity/forcetest.c:24-174): FP64 direct sum
+        (+ Ewald correction per pair when periodic) for particles idx in the
+        current (PH) order."""
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        out = np.zeros((idx.size, 3))
+        self._ok(self.lib.tdv_forcetest(self._c, idx.size,
+                                        idx.ctypes.data_as(_vp), _d(out)),
+                 "tdv_forcetest")
+        return out
 
     # -- element-wise functions -------------------------------------------------
     def peano_keys(self, x, y, z, origin, size, reversed_=False):
