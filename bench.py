@@ -57,11 +57,12 @@ def make_workload(name):
     return pos, vel, m, soft, w
 
 
-def run_gpu(pos, m, soft, w, steps, warmup, precision, ewald_size):
+def run_gpu(pos, m, soft, w, steps, warmup, precision, ewald_size,
+            ewald_accuracy):
     from tandav_amd import Engine
     e = Engine(periodic=True, boxsize=1.0, softening=soft, theta_bh=0.1,
                alpha_rel=w["alpha"], compat=True, precision=precision,
-               ewald_size=ewald_size)
+               ewald_size=ewald_size, ewald_accuracy=ewald_accuracy)
     e.set_particles(pos, m)
     e.setup_domain()
     e.domain_decomposition()
@@ -137,7 +138,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
-    ap.add_argument("--ewald-size", type=float, default=1.0 / 64)
+    ap.add_argument("--ewald-size", type=float, default=0.125)
+    ap.add_argument("--ewald-accuracy", type=float, default=0.1)
     ap.add_argument("--error-sample", type=int, default=8192)
     ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -162,7 +164,7 @@ def main():
         if args.only and prec != args.only:
             continue
         e, t_step, stage = run_gpu(pos, m, soft, w, args.steps, args.warmup,
-                                   prec, args.ewald_size)
+                                   prec, args.ewald_size, args.ewald_accuracy)
         got = e.get("acc", "cost", "last_acc_mag")
         if ref is None:
             t0 = time.perf_counter()
@@ -230,6 +232,7 @@ def main():
         "config": {"workload": w["desc"], "n_particles": n,
                    "softening": soft, "alpha_rel": w["alpha"],
                    "ewald_size": args.ewald_size,
+                   "ewald_accuracy": args.ewald_accuracy,
                    "error_sample": len(idx), "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -57,6 +57,9 @@ typedef struct {
 	                          group walk on fixed-point separations           */
 	double ewald_size;     /* FP32 walk: largest node (box units) whose Ewald
 	                          correction is taken at its centre of mass       */
+	double ewald_accuracy; /* FP32 walk: factor on the opening parameter for
+	                          taking a node's Ewald correction at its centre
+	                          of mass (smaller = more accurate)               */
 } tdv_params;
 
 enum { TDV_WALK_FP64 = 0, TDV_WALK_FP32 = 1 };

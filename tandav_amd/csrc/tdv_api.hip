@@ -226,6 +226,9 @@ int tdv_create(const tdv_params *p, tdv_context **out)
 		return TDV_ERR_ARG;
 	if (p->precision != TDV_WALK_FP64 && p->precision != TDV_WALK_FP32)
 		return TDV_ERR_ARG;
+	if (p->precision == TDV_WALK_FP32 && p->periodic
+			&& !(p->ewald_size > 0 && p->ewald_accuracy > 0))
+		return TDV_ERR_ARG;
 	tdv_context *c = new (std::nothrow) tdv_context();
 	if (!c)
 		return TDV_ERR_NOMEM;

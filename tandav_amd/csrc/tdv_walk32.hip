@@ -18,13 +18,12 @@
  *
  * Periodic runs add the Ewald correction in the same traversal: a node that
  * is taken as a monopole adds Newton + correction at its centre of mass
- * (the periodic force of a point mass); when a node is opened and is
- * already small (<= ewald_size) and does not straddle the half box for any
- * lane, its correction is added once at its centre of mass and its subtree
- * then adds Newton terms only.  Only the thin layer of bundles that
- * straddle the half box falls back to per-particle corrections.  The
- * correction comes from the reference's 65^3 table (Gravity/periodic.c:
- * 35-113) in FP32, L2-resident.
+ * (the periodic force of a point mass); when a node is opened but every
+ * lane's Ewald criterion (see ewald_ok) accepts its centre of mass, its
+ * correction is added once there and its subtree then adds Newton terms
+ * only.  Only bundles that straddle the half box or fail that criterion
+ * fall back to per-particle corrections.  The correction comes from the
+ * reference's 65^3 table (Gravity/periodic.c:35-113) in FP32, L2-resident.
  */
 #include "tdv_internal.h"
 #include "tdv_device.h"
@@ -35,6 +34,7 @@ struct Walk32Params {
 	float eps2, inv_eps2, k_inner;   /* fixed units: eps^2, 1/eps^2,
 	                                    1/(16 eps^3) */
 	float ewald_size;                /* correction at nodes <= this size */
+	float ewald_acc;                 /* factor on the criterion for that */
 	float half;                      /* half box in fixed units */
 	float box_f;                     /* box in fixed units */
 	float u_scale;                   /* fixed units -> table grid units */
@@ -181,12 +181,34 @@ __device__ __forceinline__ float fdiff(int a, int b)
 	return (float)(int)((unsigned)a - (unsigned)b);
 }
 
-/* any lane: particle pos within the node's straddle zone (half box) */
+/* the node's straddle zone: some of its particles have another nearest
+ * image than its centre (Gravity/tree_periodic.c:280-308) */
 __device__ __forceinline__ bool straddles(float sx, float sy, float sz,
 		float s, const Walk32Params &w)
 {
 	const float lim = 0.5f * (w.box_f - s);
 	return fabsf(sx) > lim || fabsf(sy) > lim || fabsf(sz) > lim;
+}
+
+/* Can a node's Ewald correction be taken at its centre of mass?  The
+ * correction is smooth inside the nearest-image cell, but its curvature
+ * grows like that of the Newton force of the nearest other image, at
+ * distance d_img >= box - max|d_k|.  The monopole error is therefore bounded
+ * like the tree's own quadrupole error with d_img in place of r: the same
+ * relative criterion (M s^2 <= d_img^4 |a| alpha / G) or BH criterion
+ * (s^2 <= theta d_img^2) decides, plus the straddle test and a size cap. */
+__device__ __forceinline__ bool ewald_ok(float m, float s, float rx, float ry,
+		float rz, float sx, float sy, float sz, float crit,
+		const Walk32Params &w)
+{
+	if (s > w.ewald_size || straddles(sx, sy, sz, s, w))
+		return false;
+	const float dimg = w.box_f - fmaxf(fabsf(rx), fmaxf(fabsf(ry),
+			fabsf(rz)));
+	const float d2 = dimg * dimg;
+	if (w.use_bh)
+		return s * s <= (w.theta_bh * w.ewald_acc) * d2;
+	return m * s * s <= (d2 * (crit * w.ewald_acc)) * d2;
 }
 
 __global__ void __launch_bounds__(64) k_walk32(int nact,
@@ -267,20 +289,20 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 				const int f = -c.x - 1, l = f + c.y;
 				bool per_particle = false;
 				if (pending) {
-					/* the bundle's correction at its CoM when it is small
-					 * and straddles the half box for no lane, else per
-					 * particle */
+					/* the bundle's correction at its CoM when every lane
+					 * allows it, else per particle */
 					const int4 A = a32[node];
 					const int4 B = b32[node];
-					const float s = __int_as_float(B.w);
-					const bool st = straddles(fdiff(B.x, me.x),
-							fdiff(B.y, me.y), fdiff(B.z, me.z), s, w);
-					if (s > w.ewald_size || __any(st)) {
+					const float s = __int_as_float(B.w),
+							m = __int_as_float(A.w);
+					const float rx = fdiff(A.x, me.x), ry = fdiff(A.y, me.y),
+							rz = fdiff(A.z, me.z);
+					const bool ok = ewald_ok(m, s, rx, ry, rz, fdiff(B.x, me.x),
+							fdiff(B.y, me.y), fdiff(B.z, me.z), crit, w);
+					if (__all(ok))
+						ewald32(a, tab, m, rx, ry, rz, w);
+					else
 						per_particle = true;
-					} else {
-						ewald32(a, tab, __int_as_float(A.w), fdiff(A.x, me.x),
-								fdiff(A.y, me.y), fdiff(A.z, me.z), w);
-					}
 				}
 				for (int q = f; q < l; q++) {
 					const int4 pq = p32[q];
@@ -319,8 +341,8 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 				node += c.x;
 				continue;
 			}
-			if (pending && s <= w.ewald_size
-					&& !__any(straddles(sx, sy, sz, s, w))) {
+			if (pending && __all(ewald_ok(m, s, rx, ry, rz, sx, sy, sz, crit,
+					w))) {
 				ewald32(a, tab, m, rx, ry, rz, w);
 				ew_until = node + c.x;
 			}
@@ -401,6 +423,7 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	w.box_f = (float)(c->par.periodic ? 4294967296.0 : 2147483648.0);
 	w.half = 0.5f * w.box_f;
 	w.ewald_size = (float)(c->par.ewald_size * c->par.boxsize * S);
+	w.ewald_acc = (float)c->par.ewald_accuracy;
 	w.u_scale = (float)(2.0 * TDV_NE / 4294967296.0);
 	w.carry_every = 64;
 	hipLaunchKernelGGL(k_walk32, dim3(tdv_blocks((size_t)c->nactive, 64)),

@@ -34,7 +34,8 @@ class TdvParams(ctypes.Structure):
                 ("theta_bh", ctypes.c_double), ("alpha_rel", ctypes.c_double),
                 ("vector_size", ctypes.c_int), ("compat", ctypes.c_int),
                 ("device", ctypes.c_int), ("precision", ctypes.c_int),
-                ("ewald_size", ctypes.c_double)]
+                ("ewald_size", ctypes.c_double),
+                ("ewald_accuracy", ctypes.c_double)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -109,14 +110,14 @@ class Engine:
     def __init__(self, periodic=False, boxsize=1.0, gravity=1.0,
                  softening=0.01, theta_bh=0.1, alpha_rel=0.02, vector_size=64,
                  compat=True, device=0, ewald_table=None, precision="fp64",
-                 ewald_size=0.0625):
+                 ewald_size=0.125, ewald_accuracy=0.1):
         self.lib = load_library()
         prec = {"fp64": 0, "fp32": 1}[precision]
         self.params = TdvParams(int(periodic), float(boxsize), float(gravity),
                                 float(softening), float(theta_bh),
                                 float(alpha_rel), int(vector_size),
                                 int(compat), int(device), prec,
-                                float(ewald_size))
+                                float(ewald_size), float(ewald_accuracy))
         self._c = _ctx()
         rc = self.lib.tdv_create(ctypes.byref(self.params),
                                  ctypes.byref(self._c))
