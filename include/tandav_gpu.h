@@ -99,6 +99,21 @@ int tdv_setup_domain(tdv_context *ctx);
  * active list (Make_Active_Particle_List, timestep.c:280-299). */
 int tdv_domain_decomposition(tdv_context *ctx);
 
+/* The Domain (domain.h:58-63) the caller computed, for the next
+ * tdv_sort_particles. */
+int tdv_set_domain(tdv_context *ctx, double size, const double origin[3]);
+
+/* Sort_Particles_By_Peano_Key (peano.c:20-48) with the domain given to
+ * tdv_set_domain: keys, key sort, reorder of every column. */
+int tdv_sort_particles(tdv_context *ctx);
+
+/* D[].TNode (domain.h:40-52) from the caller's own decomposition
+ * (transform_bunches_into_top_nodes, domain.c:220-250): ntop 64-byte
+ * records in descending key order whose [First_Part, First_Part+Npart)
+ * ranges tile the particles; Key, Level, First_Part, Npart and Pos are read.
+ * Replaces the NTarget = 1 top nodes of tdv_domain_decomposition. */
+int tdv_set_top_nodes(tdv_context *ctx, const void *records, int ntop);
+
 /* Gravity_Tree_Build (Gravity/tree_build.c:47-122) from the sorted keys. */
 int tdv_tree_build(tdv_context *ctx);
 

@@ -105,6 +105,7 @@ def _declare(L):
     for f in ("orc_sim_setup_domain", "orc_sim_domain_decomposition",
               "orc_sim_tree_build"):
         getattr(L, f).argtypes = [_vp]
+    L.orc_sim_set_top_nodes.argtypes = [_vp, _vp, ctypes.c_int]
     L.orc_sim_tree_acceleration.argtypes = [_vp, ctypes.c_int]
     L.orc_sim_compute_acceleration.argtypes = [_vp, ctypes.c_int]
     L.orc_sim_ewald_correction.argtypes = [_vp, _dp, _dp]
@@ -251,6 +252,10 @@ class Sim:
 
     def tree_build(self):
         lib().orc_sim_tree_build(self._s)
+
+    def set_top_nodes(self, rec):
+        rec = np.ascontiguousarray(rec, dtype=TOP_DTYPE)
+        lib().orc_sim_set_top_nodes(self._s, rec.ctypes.data_as(_vp), len(rec))
 
     def tree_acceleration(self, use_bh):
         lib().orc_sim_tree_acceleration(self._s, int(use_bh))

@@ -138,6 +138,7 @@ int orc_sim_nactive(const orc_sim *s);
 const int *orc_sim_active(const orc_sim *s);
 void orc_sim_setup_domain(orc_sim *s);          /* Setup_Domain_Decomposition */
 void orc_sim_domain_decomposition(orc_sim *s);  /* Domain_Decomposition       */
+void orc_sim_set_top_nodes(orc_sim *s, const orc_topnode *rec, int ntop);
 void orc_sim_tree_build(orc_sim *s);            /* Gravity_Tree_Build         */
 void orc_sim_tree_acceleration(orc_sim *s, int use_bh); /* + periodic part */
 void orc_sim_compute_acceleration(orc_sim *s, int prepare_step);

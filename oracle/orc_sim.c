@@ -312,8 +312,34 @@ static void make_top_nodes(orc_sim *s)
 					+ s->dom_origin[d]);
 		}
 	}
+	free(s->D);
+	s->D = calloc(64, sizeof(*s->D));
 	memcpy(s->D, b, (size_t)nb * sizeof(b[0]));
 	s->ntop = nb;
+}
+
+/* top nodes from a caller's decomposition (transform_bunches_into_top_nodes,
+ * domain.c:220-250, with the bunch list given): records in descending key
+ * order; Key, Level, First_Part, Npart are read, Pos is recomputed */
+void orc_sim_set_top_nodes(orc_sim *s, const orc_topnode *rec, int ntop)
+{
+	free(s->D);
+	s->D = calloc((size_t)ntop, sizeof(*s->D));
+	for (int i = 0; i < ntop; i++) {
+		orc_topnode b = rec[i];
+		const double size = s->dom_size / (double)(1ULL << b.Level);
+		const int ip = b.First_Part;
+		for (int d = 0; d < 3; d++) {
+			double p = s->P.pos[d][ip] - s->dom_origin[d];
+			b.Pos[d] = (float)((floor(p / size) + 0.5) * size + s->dom_origin[d]);
+		}
+		b.Target = -1;
+		b.Mass = 0;
+		memset(b.CoM, 0, sizeof(b.CoM));
+		memset(b.Dp, 0, sizeof(b.Dp));
+		s->D[i] = b;
+	}
+	s->ntop = ntop;
 }
 
 void orc_sim_domain_decomposition(orc_sim *s)

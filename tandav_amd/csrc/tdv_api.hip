@@ -13,6 +13,13 @@
 
 #define TDV_EBIT_MASK (TDV_EBIT_DUP | TDV_EBIT_DEPTH | TDV_EBIT_RANGE | TDV_EBIT_TIES)
 
+/* Top_Tree_Node layout, domain.h:40-52 (records in and out) */
+struct TopRecordIn {
+	uint64_t Key;
+	int Target, Level, First_Part, Npart;
+	float Pos[3], Mass, CoM[3], Dp[3];
+};
+
 enum { EV_KEYS0, EV_KEYS1, EV_SORT1, EV_REORDER1, EV_TOP1, EV_BUILD0,
 	EV_BUILD1, EV_WALK0, EV_WALK1, EV_EWALD1, EV_KICK0, EV_KICK1, EV_N };
 
@@ -100,6 +107,7 @@ static void free_particles(tdv_context *c)
 	dfree(c->cnt);
 	dfree(c->off);
 	dfree(c->lohi);
+	dfree(c->top.ptop);
 	c->cap = 0;
 }
 
@@ -137,6 +145,7 @@ static int alloc_particles(tdv_context *c, size_t n)
 	TDV_TRY(dalloc(c, &c->cnt, n));
 	TDV_TRY(dalloc(c, &c->off, n + 1));
 	TDV_TRY(dalloc(c, &c->lohi, n));
+	TDV_TRY(dalloc(c, &c->top.ptop, n));
 	c->cap = n;
 	return 0;
 }
@@ -159,25 +168,6 @@ int tdv_ensure_nodes(tdv_context *c, size_t nodes)
 	return 0;
 }
 
-static int alloc_top(tdv_context *c)
-{
-	TopNodes &t = c->top;
-	TDV_TRY(dalloc(c, &t.first, TDV_MAX_TOP));
-	TDV_TRY(dalloc(c, &t.npart, TDV_MAX_TOP));
-	TDV_TRY(dalloc(c, &t.level, TDV_MAX_TOP));
-	TDV_TRY(dalloc(c, &t.target, TDV_MAX_TOP));
-	TDV_TRY(dalloc(c, &t.term, TDV_MAX_TOP));
-	TDV_TRY(dalloc(c, &t.key, TDV_MAX_TOP));
-	TDV_TRY(dalloc(c, &t.pos, 3 * TDV_MAX_TOP));
-	TDV_TRY(dalloc(c, &t.mass, TDV_MAX_TOP));
-	TDV_TRY(dalloc(c, &t.com, 3 * TDV_MAX_TOP));
-	TDV_TRY(dalloc(c, &t.cell2top, 64));
-	TDV_TRY(dalloc(c, &t.ntop, 1));
-	TDV_TRY(dalloc(c, &t.nnodes, 1));
-	TDV_HIP(c, hipMemset(t.ntop, 0, sizeof(int)));
-	return 0;
-}
-
 static void free_top(tdv_context *c)
 {
 	TopNodes &t = c->top;
@@ -190,9 +180,31 @@ static void free_top(tdv_context *c)
 	dfree(t.pos);
 	dfree(t.mass);
 	dfree(t.com);
-	dfree(t.cell2top);
 	dfree(t.ntop);
 	dfree(t.nnodes);
+	c->top_cap = 0;
+}
+
+static int alloc_top(tdv_context *c, int cap)
+{
+	int *ptop = c->top.ptop;        /* per particle, owned by the columns */
+	free_top(c);
+	c->top.ptop = ptop;
+	TopNodes &t = c->top;
+	TDV_TRY(dalloc(c, &t.first, cap));
+	TDV_TRY(dalloc(c, &t.npart, cap));
+	TDV_TRY(dalloc(c, &t.level, cap));
+	TDV_TRY(dalloc(c, &t.target, cap));
+	TDV_TRY(dalloc(c, &t.term, cap));
+	TDV_TRY(dalloc(c, &t.key, cap));
+	TDV_TRY(dalloc(c, &t.pos, 3 * (size_t)cap));
+	TDV_TRY(dalloc(c, &t.mass, cap));
+	TDV_TRY(dalloc(c, &t.com, 3 * (size_t)cap));
+	TDV_TRY(dalloc(c, &t.ntop, 1));
+	TDV_TRY(dalloc(c, &t.nnodes, 1));
+	TDV_HIP(c, hipMemset(t.ntop, 0, sizeof(int)));
+	c->top_cap = cap;
+	return 0;
 }
 
 static int compute_ewald(tdv_context *c)
@@ -250,7 +262,7 @@ int tdv_create(const tdv_params *p, tdv_context **out)
 	if (!r)
 		r = tdv_check(c, hipMemset(c->d_err, 0, sizeof(unsigned)), "memset");
 	if (!r)
-		r = alloc_top(c);
+		r = alloc_top(c, 64);
 	if (!r && p->periodic)
 		r = compute_ewald(c);       /* Gravity_Periodic_Init */
 	if (r) {
@@ -279,6 +291,7 @@ void tdv_destroy(tdv_context *c)
 	dfree(c->p32);
 	dfree(c->a32);
 	dfree(c->b32);
+	dfree(c->top32);
 	dfree(c->d_err);
 	dfree(c->d_count);
 	for (int e = 0; e < EV_N; e++)
@@ -408,12 +421,10 @@ int tdv_get_particles(tdv_context *c, double *pos[3], double *vel[3],
 	return tdv_check(c, hipStreamSynchronize(c->stream), "get_particles");
 }
 
-/* domain + keys + sort + reorder of every column (the part shared by
- * Setup_Domain_Decomposition and Domain_Decomposition) */
-static int domain_keys_sort(tdv_context *c)
+/* set_computational_domain (domain.c:732-847) */
+static int computational_domain(tdv_context *c)
 {
-	const size_t n = c->n;
-	if (!n)
+	if (!c->n)
 		return tdv_fail(c, TDV_ERR_ARG, "no particles");
 	if (c->par.periodic) {                 /* domain.c:732-747 */
 		c->dom_size = c->par.boxsize;
@@ -421,9 +432,18 @@ static int domain_keys_sort(tdv_context *c)
 			c->dom_center[d] = 0.5 * c->dom_size;
 			c->dom_origin[d] = 0;
 		}
-	} else {
-		TDV_TRY(tdv_domain_nonperiodic(c));
+		return 0;
 	}
+	return tdv_domain_nonperiodic(c);
+}
+
+/* Sort_Particles_By_Peano_Key (peano.c:20-48) with the current domain:
+ * keys, sort, reorder of every column */
+static int keys_sort_reorder(tdv_context *c)
+{
+	const size_t n = c->n;
+	if (!n)
+		return tdv_fail(c, TDV_ERR_ARG, "no particles");
 	(void)hipEventRecord(c->ev[EV_KEYS0], c->stream);
 	tdv_launch_keys(c, n, c->pos[0], c->pos[1], c->pos[2], c->dom_origin,
 			c->dom_size, c->key_tmp, 0);
@@ -455,22 +475,97 @@ static int domain_keys_sort(tdv_context *c)
 	TDV_TRY(tdv_check_err_flags(c));
 	c->have_domain = 1;
 	c->have_tree = 0;
+	c->have_top = 0;
 	return 0;
+}
+
+int tdv_set_domain(tdv_context *c, double size, const double origin[3])
+{
+	if (!c || !origin || !(size > 0))
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_set_domain: bad arguments");
+	c->dom_size = size;
+	for (int d = 0; d < 3; d++) {
+		c->dom_origin[d] = origin[d];
+		c->dom_center[d] = origin[d] + 0.5 * size;
+	}
+	c->have_domain_set = 1;
+	return 0;
+}
+
+int tdv_sort_particles(tdv_context *c)
+{
+	if (!c || !c->have_domain_set)
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_sort_particles before a domain");
+	return keys_sort_reorder(c);
 }
 
 int tdv_setup_domain(tdv_context *c)
 {
 	if (!c)
 		return TDV_ERR_ARG;
-	return domain_keys_sort(c);
+	TDV_TRY(computational_domain(c));
+	c->have_domain_set = 1;
+	return keys_sort_reorder(c);
+}
+
+int tdv_set_top_nodes(tdv_context *c, const void *records, int ntop)
+{
+	if (!c || !records || ntop <= 0 || ntop > TDV_MAX_TOP || !c->have_domain)
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_set_top_nodes: bad arguments");
+	const TopRecordIn *r = static_cast<const TopRecordIn *>(records);
+	/* contiguous particle ranges in descending key order covering all */
+	size_t expect = c->n;
+	for (int t = 0; t < ntop; t++) {
+		if (r[t].Npart <= 0 || r[t].First_Part < 0 || r[t].Level < 1
+				|| r[t].Level > TDV_NT - 1
+				|| (size_t)r[t].First_Part + (size_t)r[t].Npart != expect)
+			return tdv_fail(c, TDV_ERR_ARG, "tdv_set_top_nodes: top nodes must "
+					"tile the particles in descending key order");
+		expect = (size_t)r[t].First_Part;
+	}
+	if (expect != 0)
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_set_top_nodes: particles left over");
+	if (ntop > c->top_cap)
+		TDV_TRY(alloc_top(c, ntop));
+	std::vector<int> first(ntop), npart(ntop), level(ntop), target(ntop, -1);
+	std::vector<uint64_t> key(ntop);
+	std::vector<float> pos(3 * (size_t)ntop);
+	for (int t = 0; t < ntop; t++) {
+		first[t] = r[t].First_Part;
+		npart[t] = r[t].Npart;
+		level[t] = r[t].Level;
+		key[t] = r[t].Key;
+		for (int d = 0; d < 3; d++)
+			pos[3 * t + d] = r[t].Pos[d];
+	}
+	auto up = [&](void *d, const void *h, size_t b) {
+		return tdv_check(c, hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice,
+				c->stream), "top copy");
+	};
+	TDV_TRY(up(c->top.first, first.data(), ntop * 4));
+	TDV_TRY(up(c->top.npart, npart.data(), ntop * 4));
+	TDV_TRY(up(c->top.level, level.data(), ntop * 4));
+	TDV_TRY(up(c->top.target, target.data(), ntop * 4));
+	TDV_TRY(up(c->top.key, key.data(), ntop * 8));
+	TDV_TRY(up(c->top.pos, pos.data(), ntop * 12));
+	TDV_TRY(up(c->top.ntop, &ntop, 4));
+	c->ntop = ntop;
+	TDV_TRY(tdv_assign_top(c));
+	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	c->have_top = 1;
+	c->have_tree = 0;
+	return 0;
 }
 
 int tdv_domain_decomposition(tdv_context *c)
 {
 	if (!c)
 		return TDV_ERR_ARG;
-	TDV_TRY(domain_keys_sort(c));
+	TDV_TRY(computational_domain(c));
+	c->have_domain_set = 1;
+	TDV_TRY(keys_sort_reorder(c));
 	TDV_TRY(tdv_build_top_nodes(c));
+	c->have_top = 1;
 	(void)hipEventRecord(c->ev[EV_TOP1], c->stream);
 	TDV_HIP(c, hipStreamSynchronize(c->stream));
 	float ms;
@@ -487,8 +582,8 @@ int tdv_domain_decomposition(tdv_context *c)
 
 int tdv_tree_build(tdv_context *c)
 {
-	if (!c || !c->have_domain)
-		return tdv_fail(c, TDV_ERR_ARG, "tree build before a decomposition");
+	if (!c || !c->have_domain || !c->have_top)
+		return tdv_fail(c, TDV_ERR_ARG, "tree build before top nodes exist");
 	(void)hipEventRecord(c->ev[EV_BUILD0], c->stream);
 	TDV_TRY(tdv_build_tree(c));
 	(void)hipEventRecord(c->ev[EV_BUILD1], c->stream);
@@ -709,7 +804,7 @@ struct TopRecord {
 
 int tdv_get_top_nodes(tdv_context *c, void *out, int max, int *ntop)
 {
-	if (!c || !out || !ntop || !c->have_domain)
+	if (!c || !out || !ntop || !c->have_top)
 		return tdv_fail(c, TDV_ERR_ARG, "no top nodes yet");
 	const int n = c->ntop;
 	*ntop = n;

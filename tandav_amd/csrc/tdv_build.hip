@@ -67,11 +67,8 @@ __global__ void __launch_bounds__(64) k_top_nodes(size_t n,
 	}
 	if (cell == 0)
 		*t.ntop = total;
-	if (np == 0) {
-		t.cell2top[cell] = -1;
+	if (np == 0)
 		return;
-	}
-	t.cell2top[cell] = rank;
 	t.first[rank] = (int)f;
 	t.npart[rank] = np;
 	t.level[rank] = TDV_TOP_LEVEL;
@@ -84,6 +81,31 @@ __global__ void __launch_bounds__(64) k_top_nodes(size_t n,
 	t.target[rank] = -1;
 }
 
+/* per particle: the top node holding it (top nodes are contiguous particle
+ * ranges in descending key order, i.e. descending first particle) */
+__global__ void k_assign_top(size_t n, TopNodes t, int ntop)
+{
+	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	int lo = 0, hi = ntop - 1;          /* first[] decreases with index */
+	while (lo < hi) {
+		const int mid = (lo + hi) >> 1;
+		if ((size_t)t.first[mid] > i)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	t.ptop[i] = lo;
+}
+
+int tdv_assign_top(tdv_context *c)
+{
+	hipLaunchKernelGGL(k_assign_top, dim3(tdv_blocks(c->n, 256)), dim3(256), 0,
+			c->stream, c->n, c->top, c->ntop);
+	return tdv_check(c, hipGetLastError(), "k_assign_top");
+}
+
 int tdv_build_top_nodes(tdv_context *c)
 {
 	hipLaunchKernelGGL(k_top_nodes, dim3(1), dim3(64), 0, c->stream, c->n,
@@ -93,7 +115,7 @@ int tdv_build_top_nodes(tdv_context *c)
 	TDV_HIP(c, hipMemcpyAsync(&c->ntop, c->top.ntop, sizeof(int),
 			hipMemcpyDeviceToHost, c->stream));
 	TDV_HIP(c, hipStreamSynchronize(c->stream));
-	return 0;
+	return tdv_assign_top(c);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -102,26 +124,29 @@ int tdv_build_top_nodes(tdv_context *c)
 
 struct TopRange {
 	int t, first, last;     /* top node, its particle range */
+	int level;              /* top node level */
 	bool tree;              /* top node holds > VECTOR_SIZE particles */
 };
 
 __device__ __forceinline__ TopRange top_of(const uint64_t *key, size_t i,
 		const TopNodes &T, int vs)
 {
-	const int cell = (int)(key[2 * i + 1] >> 58);
+	(void)key;
 	TopRange r;
-	r.t = T.cell2top[cell];
+	r.t = T.ptop[i];
 	r.first = T.first[r.t];
 	r.last = r.first + T.npart[r.t] - 1;
+	r.level = T.level[r.t];
 	r.tree = T.npart[r.t] > vs;
 	return r;
 }
 
 /* dprev of particle i: D(i-1, i), the top level for the first particle */
-__device__ __forceinline__ int dprev_of(const uint64_t *key, int i, int first)
+__device__ __forceinline__ int dprev_of(const uint64_t *key, int i, int first,
+		int top_level)
 {
 	if (i == first)
-		return TDV_TOP_LEVEL;
+		return top_level;
 	return div_level(load_key(key, i - 1), load_key(key, i));
 }
 
@@ -153,12 +178,12 @@ __global__ void __launch_bounds__(256) k_tree_marks(size_t n,
 	const TopRange r = top_of(key, gi, T, vs);
 	if (!r.tree || i == r.first)
 		return;
-	const int dp = dprev_of(key, i, r.first);
+	const int dp = dprev_of(key, i, r.first, r.level);
 	if (dp >= TDV_NT) {          /* below key resolution: not supported */
 		atomicOr(err, TDV_EBIT_DEPTH);
 		return;
 	}
-	const int dpp = dprev_of(key, i - 1, r.first);
+	const int dpp = dprev_of(key, i - 1, r.first, r.level);
 	const bool newb = (i >= r.first + 2) && dp < dpp;
 	if (!(newb || i == r.last))
 		return;
@@ -196,7 +221,7 @@ __global__ void __launch_bounds__(256) k_tree_counts(size_t n,
 		lohi[i] = 0;
 		return;
 	}
-	const int lo = dprev_of(key, i, r.first);
+	const int lo = dprev_of(key, i, r.first, r.level);
 	const int dn = (i == r.last) ? -1 : div_level(load_key(key, i),
 			load_key(key, i + 1));
 	const int lf = lo > dn ? lo : dn;
@@ -209,11 +234,10 @@ __global__ void __launch_bounds__(256) k_tree_counts(size_t n,
 }
 
 __global__ void __launch_bounds__(64) k_tree_targets(TopNodes T, int vs,
-		const uint32_t *__restrict__ off)
+		const uint32_t *__restrict__ off, int ntop)
 {
 	if (threadIdx.x != 0)
 		return;
-	const int ntop = *T.ntop;
 	int run = 0;
 	for (int t = 0; t < ntop; t++) {
 		int sz = 0;
@@ -317,10 +341,10 @@ __global__ void __launch_bounds__(256) k_tree_nodes(size_t n,
 __global__ void k_tree_small_top(const double *__restrict__ x,
 		const double *__restrict__ y, const double *__restrict__ z,
 		const double *__restrict__ m, TopNodes T, int vs,
-		const double4 *__restrict__ na, int *__restrict__ tree_parent)
+		const double4 *__restrict__ na, int *__restrict__ tree_parent, int ntop)
 {
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= *T.ntop)
+	if (t >= ntop)
 		return;
 	const int f = T.first[t], np = T.npart[t];
 	double cx, cy, cz, mm;
@@ -351,10 +375,11 @@ __global__ void k_tree_small_top(const double *__restrict__ x,
 }
 
 __global__ void k_tree_terms(TopNodes T, int vs, double4 *__restrict__ na,
-		double4 *__restrict__ nb, int4 *__restrict__ nc, int *__restrict__ up)
+		double4 *__restrict__ nb, int4 *__restrict__ nc, int *__restrict__ up,
+		int ntop)
 {
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= *T.ntop || T.npart[t] <= vs)
+	if (t >= ntop || T.npart[t] <= vs)
 		return;
 	const int j = T.term[t];
 	na[j] = make_double4(0, 0, 0, 1);   /* terminator: Mass = 1, TOP bit */
@@ -367,7 +392,7 @@ __global__ void __launch_bounds__(256) k_tree_centres(size_t nnodes,
 		const double *__restrict__ x, const double *__restrict__ y,
 		const double *__restrict__ z, TopNodes T, double dom_size,
 		const int4 *__restrict__ nc, const int *__restrict__ up,
-		double4 *__restrict__ nb, size_t npart, unsigned *err)
+		double4 *__restrict__ nb, size_t npart, unsigned *err, int ntop)
 {
 	const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (j >= nnodes)
@@ -390,7 +415,7 @@ __global__ void __launch_bounds__(256) k_tree_centres(size_t nnodes,
 		bits = nc[q].z;
 	}
 	const int t = up[q];                 /* the root's DUp is its top node */
-	if (t < 0 || t >= *T.ntop) {
+	if (t < 0 || t >= ntop) {
 		atomicOr(err, TDV_EBIT_RANGE);
 		return;
 	}
@@ -426,7 +451,7 @@ int tdv_build_tree(tdv_context *c)
 			reinterpret_cast<uint32_t *>(c->cnt), c->lohi);
 	TDV_TRY(tdv_scan_u32(c, reinterpret_cast<uint32_t *>(c->cnt), c->off, n));
 	hipLaunchKernelGGL(k_tree_targets, dim3(1), dim3(64), 0, c->stream, c->top,
-			vs, c->off);
+			vs, c->off, c->ntop);
 	TDV_HIP(c, hipGetLastError());
 	int nn = 0;
 	TDV_HIP(c, hipMemcpyAsync(&nn, c->top.nnodes, sizeof(int),
@@ -442,16 +467,17 @@ int tdv_build_tree(tdv_context *c)
 			c->stream, n, c->key, c->pos[0], c->pos[1], c->pos[2], c->mass,
 			c->top, vs, reinterpret_cast<uint32_t *>(c->cnt), c->off, c->lohi,
 			c->node_a, c->node_c, c->node_up, c->tree_parent);
-	hipLaunchKernelGGL(k_tree_terms, dim3(1), dim3(64), 0, c->stream, c->top,
-			vs, c->node_a, c->node_b, c->node_c, c->node_up);
-	hipLaunchKernelGGL(k_tree_small_top, dim3(1), dim3(64), 0, c->stream,
-			c->pos[0], c->pos[1], c->pos[2], c->mass, c->top, vs, c->node_a,
-			c->tree_parent);
+	hipLaunchKernelGGL(k_tree_terms, dim3(tdv_blocks((size_t)c->ntop, 64)),
+			dim3(64), 0, c->stream, c->top, vs, c->node_a, c->node_b, c->node_c,
+			c->node_up, c->ntop);
+	hipLaunchKernelGGL(k_tree_small_top, dim3(tdv_blocks((size_t)c->ntop, 64)),
+			dim3(64), 0, c->stream, c->pos[0], c->pos[1], c->pos[2], c->mass,
+			c->top, vs, c->node_a, c->tree_parent, c->ntop);
 	if (nn > 0)
 		hipLaunchKernelGGL(k_tree_centres, dim3(tdv_blocks((size_t)nn, 256)),
 				dim3(256), 0, c->stream, (size_t)nn, c->pos[0], c->pos[1],
 				c->pos[2], c->top, c->dom_size, c->node_c, c->node_up,
-				c->node_b, n, c->d_err);
+				c->node_b, n, c->d_err, c->ntop);
 	TDV_HIP(c, hipGetLastError());
 	c->have_tree = 1;
 	return 0;

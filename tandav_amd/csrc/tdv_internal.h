@@ -29,7 +29,7 @@
 #define TDV_NE3 (TDV_NE1 * TDV_NE1 * TDV_NE1)
 #define TDV_BIT_LOCAL 9
 #define TDV_BIT_TOP 10
-#define TDV_MAX_TOP 64
+#define TDV_MAX_TOP 65536       /* top-node records a context accepts */
 
 /* error bits raised by kernels into ctx->d_err */
 #define TDV_EBIT_DUP 1u
@@ -37,13 +37,13 @@
 #define TDV_EBIT_RANGE 4u
 #define TDV_EBIT_TIES 8u
 
-struct TopNodes {            /* device arrays of length TDV_MAX_TOP */
+struct TopNodes {            /* device arrays, top_cap entries each */
 	int *first, *npart, *level, *target, *term;
 	uint64_t *key;
 	float *pos;              /* 3 per node */
 	float *mass;
 	float *com;              /* 3 per node */
-	int *cell2top;           /* level-2 cell id -> top node index or -1 */
+	int *ptop;               /* per particle: its top node */
 	int *ntop;               /* [1] */
 	int *nnodes;             /* [1] total tree nodes */
 };
@@ -87,7 +87,9 @@ struct tdv_context {
 	int *node_up;                    /* DUp */
 
 	TopNodes top;
-	int ntop;
+	int ntop, top_cap;
+	int4 *top32;                     /* FP32 walk: fixed-point top CoM */
+	int top32_cap;
 
 	double *ewald;                   /* 4 * TDV_NE3, normalised */
 	int have_ewald;
@@ -101,6 +103,7 @@ struct tdv_context {
 	unsigned long long *d_count;     /* [2] interactions: tree, Ewald */
 	unsigned long long count[2];
 	int have_domain, have_tree;
+	int have_domain_set, have_top;
 
 	hipEvent_t ev[16];
 	float stage_ms[8];
@@ -127,6 +130,7 @@ int tdv_scan_u32(tdv_context *c, const uint32_t *in, uint32_t *out,
 
 /* ---- launchers (tdv_build.hip) ---- */
 int tdv_build_top_nodes(tdv_context *c);
+int tdv_assign_top(tdv_context *c);
 int tdv_build_tree(tdv_context *c);
 
 /* ---- launchers (tdv_walk64.hip, tdv_ewald.hip) ---- */

@@ -37,14 +37,9 @@ struct Walk32Params {
 	float ewald_acc;                 /* factor on the criterion for that */
 	float half;                      /* half box in fixed units */
 	float box_f;                     /* box in fixed units */
+	float dom_f;                     /* domain size in fixed units */
 	float u_scale;                   /* fixed units -> table grid units */
 	int carry_every;
-};
-
-struct Top32 {
-	int4 com[TDV_MAX_TOP];      /* fixed CoM, scaled mass bits */
-	int4 cen[TDV_MAX_TOP];      /* fixed centre, size bits */
-	int target[TDV_MAX_TOP], npart[TDV_MAX_TOP], first[TDV_MAX_TOP];
 };
 
 /* ---- preparation: fixed-point particle / node records -------------------- */
@@ -80,6 +75,18 @@ __global__ void k_prep32_nodes(size_t nn, const double4 *__restrict__ na,
 			(int)fixp(a.z, o2, S), __float_as_int((float)(a.w * mscale)));
 	b32[j] = make_int4((int)fixp(b.x, o0, S), (int)fixp(b.y, o1, S),
 			(int)fixp(b.z, o2, S), __float_as_int((float)(b.w * S)));
+}
+
+__global__ void k_prep32_top(int ntop, const TopNodes T, double o0, double o1,
+		double o2, double S, double mscale, int4 *__restrict__ t32)
+{
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= ntop)
+		return;
+	const double m = (double)T.mass[t] * mscale;
+	t32[t] = make_int4((int)fixp(T.com[3 * t], o0, S),
+			(int)fixp(T.com[3 * t + 1], o1, S), (int)fixp(T.com[3 * t + 2], o2, S),
+			__float_as_int((float)m));
 }
 
 __global__ void k_prep32_ewald(const double *__restrict__ raw,
@@ -219,21 +226,9 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 		Walk32Params w, double *__restrict__ ax, double *__restrict__ ay,
 		double *__restrict__ az, double *__restrict__ gx,
 		double *__restrict__ gy, double *__restrict__ gz,
-		double *__restrict__ cost, unsigned long long *counter, double o0,
-		double o1, double o2, double S, double mscale)
+		double *__restrict__ cost, unsigned long long *counter,
+		const int4 *__restrict__ top32)
 {
-	__shared__ Top32 TS;
-	for (int t = threadIdx.x; t < w.ntop; t += 64) {
-		const double m = (double)T.mass[t] * mscale;
-		TS.com[t] = make_int4((int)fixp(T.com[3 * t], o0, S),
-				(int)fixp(T.com[3 * t + 1], o1, S),
-				(int)fixp(T.com[3 * t + 2], o2, S), __float_as_int((float)m));
-		TS.target[t] = T.target[t];
-		TS.npart[t] = T.npart[t];
-		TS.first[t] = T.first[t];
-	}
-	__syncthreads();
-
 	const int gid = blockIdx.x * 64 + threadIdx.x;
 	const bool live = gid < nact;
 	const int ip = live ? active[gid] : active[nact - 1];
@@ -245,10 +240,9 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 	const int every = w.carry_every;
 
 	for (int j = 0; j < w.ntop; j++) {
-		const int4 tc = TS.com[j];
+		const int4 tc = top32[j];
 		const float tm = __int_as_float(tc.w);
-		const int lvl = TDV_TOP_LEVEL;
-		const float ts = w.box_f / (float)(1 << lvl);
+		const float ts = w.dom_f / (float)(1 << T.level[j]);
 		const float rx = fdiff(tc.x, me.x), ry = fdiff(tc.y, me.y),
 				rz = fdiff(tc.z, me.z);
 		const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
@@ -263,8 +257,8 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 			tick(a, every);
 			continue;
 		}
-		if (TS.npart[j] <= w.vs) {
-			const int f = TS.first[j], l = f + TS.npart[j];
+		if (T.npart[j] <= w.vs) {
+			const int f = T.first[j], l = f + T.npart[j];
 			for (int q = f; q < l; q++) {
 				const int4 pq = p32[q];
 				const float dx = fdiff(pq.x, me.x), dy = fdiff(pq.y, me.y),
@@ -277,7 +271,7 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 			}
 			continue;
 		}
-		const int start = TS.target[j];
+		const int start = T.target[j];
 		int node = start;
 		int ew_until = w.periodic ? -1 : 0x7FFFFFFF;  /* correction done below */
 		for (;;) {
@@ -397,6 +391,15 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	}
 	hipLaunchKernelGGL(k_prep32_particles, dim3(tdv_blocks(n, 256)), dim3(256),
 			0, c->stream, n, c->pm, o[0], o[1], o[2], S, mscale, c->p32);
+	if (c->top32_cap < c->top_cap) {
+		if (c->top32)
+			(void)hipFree(c->top32);
+		TDV_HIP(c, hipMalloc((void **)&c->top32, c->top_cap * sizeof(int4)));
+		c->top32_cap = c->top_cap;
+	}
+	hipLaunchKernelGGL(k_prep32_top, dim3(tdv_blocks((size_t)c->ntop, 64)),
+			dim3(64), 0, c->stream, c->ntop, c->top, o[0], o[1], o[2], S,
+			mscale, c->top32);
 	if (nn)
 		hipLaunchKernelGGL(k_prep32_nodes, dim3(tdv_blocks(nn, 256)),
 				dim3(256), 0, c->stream, nn, c->node_a, c->node_b, o[0], o[1],
@@ -421,6 +424,7 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	w.inv_eps2 = (float)(1.0 / (ef * ef));
 	w.k_inner = (float)(1.0 / (16.0 * ef * ef * ef));
 	w.box_f = (float)(c->par.periodic ? 4294967296.0 : 2147483648.0);
+	w.dom_f = (float)(c->dom_size * S);
 	w.half = 0.5f * w.box_f;
 	w.ewald_size = (float)(c->par.ewald_size * c->par.boxsize * S);
 	w.ewald_acc = (float)c->par.ewald_accuracy;
@@ -430,6 +434,6 @@ int tdv_walk32(tdv_context *c, int use_bh)
 			dim3(64), 0, c->stream, c->nactive, c->active, c->p32, c->last_acc,
 			c->top, c->a32, c->b32, c->node_c, c->ewald32, w, c->acc[0],
 			c->acc[1], c->acc[2], c->gacc[0], c->gacc[1], c->gacc[2], c->cost,
-			c->d_count, o[0], o[1], o[2], S, mscale);
+			c->d_count, c->top32);
 	return tdv_check(c, hipGetLastError(), "k_walk32");
 }

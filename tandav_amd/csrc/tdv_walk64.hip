@@ -21,30 +21,6 @@ struct WalkParams {
 	double dom_size, box2ewald;
 };
 
-struct TopShared {
-	float pos[TDV_MAX_TOP * 3], com[TDV_MAX_TOP * 3], mass[TDV_MAX_TOP];
-	int npart[TDV_MAX_TOP], target[TDV_MAX_TOP], first[TDV_MAX_TOP];
-	int level[TDV_MAX_TOP], term[TDV_MAX_TOP];
-};
-
-__device__ __forceinline__ void load_top(TopShared &s, const TopNodes &T,
-		int ntop)
-{
-	for (int t = threadIdx.x; t < ntop; t += blockDim.x) {
-		for (int d = 0; d < 3; d++) {
-			s.pos[3 * t + d] = T.pos[3 * t + d];
-			s.com[3 * t + d] = T.com[3 * t + d];
-		}
-		s.mass[t] = T.mass[t];
-		s.npart[t] = T.npart[t];
-		s.target[t] = T.target[t];
-		s.first[t] = T.first[t];
-		s.level[t] = T.level[t];
-		s.term[t] = T.term[t];
-	}
-	__syncthreads();
-}
-
 /* Periodic_Nearest (periodic.c:62-80) */
 __device__ __forceinline__ double wrap(double d, const WalkParams &w)
 {
@@ -113,8 +89,6 @@ __global__ void __launch_bounds__(256) k_walk64(int nact,
 		double *__restrict__ gz, double *__restrict__ cost, int npart_total,
 		unsigned *err, unsigned long long *counter)
 {
-	__shared__ TopShared S;
-	load_top(S, T, w.ntop);
 	const int gid = blockIdx.x * blockDim.x + threadIdx.x;
 	const bool live = gid < nact;
 	const int ip = live ? active[gid] : 0;
@@ -124,19 +98,19 @@ __global__ void __launch_bounds__(256) k_walk64(int nact,
 
 	for (int j = 0; live && j < w.ntop; j++) {
 		/* interact_with_topnode, Gravity/tree_accel.c:137-176 */
-		const double nSize = w.dom_size / ((double)(1ull << S.level[j]));
+		const double nSize = w.dom_size / ((double)(1ull << T.level[j]));
 		{
-			const double dx = (double)S.pos[3 * j] - p.x;
-			const double dy = (double)S.pos[3 * j + 1] - p.y;
-			const double dz = (double)S.pos[3 * j + 2] - p.z;
+			const double dx = (double)T.pos[3 * j] - p.x;
+			const double dy = (double)T.pos[3 * j + 1] - p.y;
+			const double dz = (double)T.pos[3 * j + 2] - p.z;
 			const bool inside = fabs(dx) < 0.6 * nSize && fabs(dy) < 0.6 * nSize
 					&& fabs(dz) < 0.6 * nSize;
 			if (!inside) {
-				const double rx = wrap((double)S.com[3 * j] - p.x, w);
-				const double ry = wrap((double)S.com[3 * j + 1] - p.y, w);
-				const double rz = wrap((double)S.com[3 * j + 2] - p.z, w);
+				const double rx = wrap((double)T.com[3 * j] - p.x, w);
+				const double ry = wrap((double)T.com[3 * j + 1] - p.y, w);
+				const double rz = wrap((double)T.com[3 * j + 2] - p.z, w);
 				const double r2 = rx * rx + ry * ry + rz * rz;
-				const double m = (double)S.mass[j];
+				const double m = (double)T.mass[j];
 				const bool open = w.use_bh ? (nSize * nSize > r2 * w.theta_bh)
 						: (m * nSize * nSize > r2 * r2 * fac);
 				if (!open) {
@@ -145,18 +119,18 @@ __global__ void __launch_bounds__(256) k_walk64(int nact,
 				}
 			}
 		}
-		if (S.npart[j] <= w.vs) {
+		if (T.npart[j] <= w.vs) {
 			/* interact_with_topnode_particles, tree_accel.c:183-203 */
-			const int f = w.compat ? S.target[j] : S.first[j];
-			if (f < 0 || f + S.npart[j] > npart_total) {
+			const int f = w.compat ? T.target[j] : T.first[j];
+			if (f < 0 || f + T.npart[j] > npart_total) {
 				atomicOr(err, TDV_EBIT_RANGE);
 				continue;
 			}
-			pp_range(a, pm, f, f + S.npart[j], p.x, p.y, p.z, w);
+			pp_range(a, pm, f, f + T.npart[j], p.x, p.y, p.z, w);
 			continue;
 		}
 		/* gravity_tree_walk(_BH), Gravity/tree_accel.c:211-353 */
-		const int start = S.target[j], term = S.term[j];
+		const int start = T.target[j], term = T.term[j];
 		int node = start;
 		for (;;) {
 			if (node < start || node > term) {   /* never for a valid tree */
@@ -299,8 +273,6 @@ __global__ void __launch_bounds__(256) k_walk64_ewald(int nact,
 		double *__restrict__ cost, int npart_total, unsigned *err,
 		unsigned long long *counter)
 {
-	__shared__ TopShared S;
-	load_top(S, T, w.ntop);
 	const int gid = blockIdx.x * blockDim.x + threadIdx.x;
 	const bool live = gid < nact;
 	const int ip = live ? active[gid] : 0;
@@ -310,23 +282,23 @@ __global__ void __launch_bounds__(256) k_walk64_ewald(int nact,
 
 	for (int j = 0; live && j < w.ntop; j++) {
 		/* interact_with_topnode, Gravity/tree_periodic.c:112-177 */
-		const double s = w.dom_size / (double)(1ull << S.level[j]);
+		const double s = w.dom_size / (double)(1ull << T.level[j]);
 		bool want_open = false;
-		const double rx = wrap((double)S.com[3 * j] - p.x, w);
-		const double ry = wrap((double)S.com[3 * j + 1] - p.y, w);
-		const double rz = wrap((double)S.com[3 * j + 2] - p.z, w);
+		const double rx = wrap((double)T.com[3 * j] - p.x, w);
+		const double ry = wrap((double)T.com[3 * j + 1] - p.y, w);
+		const double rz = wrap((double)T.com[3 * j + 2] - p.z, w);
 		const double r2 = rx * rx + ry * ry + rz * rz;
 		if (w.use_bh) {
 			if (s * s > r2 * w.theta_bh)
 				want_open = true;
 		} else {
-			const double m = (double)S.mass[j];
+			const double m = (double)T.mass[j];
 			if (m * s * s > r2 * r2 * fac)
 				want_open = true;
 		}
-		if (fabs(p.x - (double)S.pos[3 * j]) < 0.6 * s
-				&& fabs(p.y - (double)S.pos[3 * j + 1]) < 0.6 * s
-				&& fabs(p.z - (double)S.pos[3 * j + 2]) < 0.6 * s)
+		if (fabs(p.x - (double)T.pos[3 * j]) < 0.6 * s
+				&& fabs(p.y - (double)T.pos[3 * j + 1]) < 0.6 * s
+				&& fabs(p.z - (double)T.pos[3 * j + 2]) < 0.6 * s)
 			want_open = true;
 		/* a node that wants opening is walked only when it straddles the
 		 * half box or is larger than 0.2 box; otherwise the correction of
@@ -338,22 +310,22 @@ __global__ void __launch_bounds__(256) k_walk64_ewald(int nact,
 					|| s > 0.2 * w.box;
 		}
 		if (!walk) {
-			ewald_cube(a, tab, rx, ry, rz, (double)S.mass[j], w);
+			ewald_cube(a, tab, rx, ry, rz, (double)T.mass[j], w);
 			continue;
 		}
-		if (S.npart[j] <= w.vs) {
-			const int f = w.compat ? S.target[j] : S.first[j];
-			if (f < 0 || f + S.npart[j] > npart_total) {
+		if (T.npart[j] <= w.vs) {
+			const int f = w.compat ? T.target[j] : T.first[j];
+			if (f < 0 || f + T.npart[j] > npart_total) {
 				atomicOr(err, TDV_EBIT_RANGE);
 				continue;
 			}
-			ewald_pp_range(a, tab, pm, f, f + S.npart[j], p.x, p.y, p.z, w);
+			ewald_pp_range(a, tab, pm, f, f + T.npart[j], p.x, p.y, p.z, w);
 			continue;
 		}
 		/* gravity_tree_walk_ewald(_BH), Gravity/tree_periodic.c:216-422 */
-		const int start = S.target[j];
+		const int start = T.target[j];
 		const int end = start + nc[start].x;
-		if (end != S.term[j]) {
+		if (end != T.term[j]) {
 			atomicOr(err, TDV_EBIT_RANGE);
 			continue;
 		}

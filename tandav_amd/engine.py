@@ -71,6 +71,9 @@ def load_library(path=LIB_PATH):
     for f in ("tdv_setup_domain", "tdv_domain_decomposition",
               "tdv_tree_build"):
         getattr(L, f).argtypes = [_ctx]
+    L.tdv_set_domain.argtypes = [_ctx, ctypes.c_double, _dp]
+    L.tdv_sort_particles.argtypes = [_ctx]
+    L.tdv_set_top_nodes.argtypes = [_ctx, _vp, ctypes.c_int]
     L.tdv_tree_acceleration.argtypes = [_ctx, ctypes.c_int]
     L.tdv_compute_acceleration.argtypes = [_ctx, ctypes.c_int]
     L.tdv_forcetest.argtypes = [_ctx, ctypes.c_int, _vp, _dp]
@@ -226,6 +229,22 @@ class Engine:
         """Domain_Decomposition (domain.c:67-145)."""
         self._ok(self.lib.tdv_domain_decomposition(self._c),
                  "tdv_domain_decomposition")
+
+    def set_domain(self, size, origin):
+        """Domain (domain.h:58-63) computed by the caller."""
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        self._ok(self.lib.tdv_set_domain(self._c, float(size), _d(o)),
+                 "tdv_set_domain")
+
+    def sort_particles(self):
+        """Sort_Particles_By_Peano_Key (peano.c:20-48) in the set domain."""
+        self._ok(self.lib.tdv_sort_particles(self._c), "tdv_sort_particles")
+
+    def set_top_nodes(self, rec):
+        """D[].TNode from the caller's decomposition (domain.c:220-250)."""
+        rec = np.ascontiguousarray(rec, dtype=TOP_DTYPE)
+        self._ok(self.lib.tdv_set_top_nodes(self._c, rec.ctypes.data_as(_vp),
+                                            len(rec)), "tdv_set_top_nodes")
 
     def tree_build(self):
         """Gravity_Tree_Build (Gravity/tree_build.c:47-122)."""

@@ -160,3 +160,53 @@ def test_config1_hernquist_65536():
     s, e = _pipeline_pair(pos, m, False, None, softening=0.01,
                           alpha_rel=0.005, nthreads=16)
     _compare(s, e)
+
+
+@pytest.mark.parametrize("case", ["clustered", "periodic"])
+def test_refined_top_nodes(case, ewald):
+    """Top nodes at mixed levels, as the reference's decomposition makes
+    with NTarget > 1, handed to both sides (tdv_set_top_nodes)."""
+    from test_oracle_tree import clustered
+    from topnodes import refined_top_nodes
+    if case == "periodic":
+        pos, _, m = ics.zeldovich_box(32, seed=7)
+        kw = dict(periodic=True, softening=1 / 1280.)
+    else:
+        pos, m = clustered(30000, 5)
+        kw = dict(periodic=False, softening=1e-3)
+    table = ewald if kw["periodic"] else None
+    s = oracle.Sim(pos, m, alpha_rel=0.005, nthreads=16, ewald_table=table,
+                   **kw)
+    s.setup_domain()
+    s.domain_decomposition()
+    size, origin, _ = s.domain
+    x = s.positions()
+    rec = refined_top_nodes(oracle.peano_keys(x[:, 0], x[:, 1], x[:, 2],
+                                              origin, size), nsplit=10)
+    s.set_top_nodes(rec)
+    s.compute_acceleration(True)
+    e = _engine(alpha_rel=0.005, compat=True, **kw)
+    if kw["periodic"]:
+        e.set_ewald_table(ewald)
+    e.set_particles(pos, m)
+    e.setup_domain()
+    e.domain_decomposition()
+    e.set_top_nodes(s.top)
+    e.compute_acceleration(True)
+    _compare(s, e)
+
+
+def test_set_domain_and_sort_particles():
+    """Sort_Particles_By_Peano_Key with the caller's Domain."""
+    pos, _, m = ics.hernquist_sphere(50000, seed=3)
+    c, size, o = oracle.domain_nonperiodic(pos[:, 0], pos[:, 1], pos[:, 2])
+    e = _engine()
+    e.set_particles(pos, m)
+    e.set_domain(size * 1.5, o - 0.25 * size)
+    e.sort_particles()
+    got = e.get("pos", "id")
+    keys = oracle.peano_keys(pos[:, 0], pos[:, 1], pos[:, 2],
+                             o - 0.25 * size, size * 1.5)
+    perm = oracle.qsort_index(keys, nthreads=4)
+    np.testing.assert_array_equal(got["id"], perm.astype(np.uint32))
+    np.testing.assert_array_equal(got["pos"], pos[perm])

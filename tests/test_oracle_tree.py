@@ -136,3 +136,36 @@ def test_parallel_rule_matches_sequential(kind, n, seed, periodic):
             assert s.tree_parent[q] == node
         ncollapsed += int(((sub["DNext"] < 0) & (sub["Npart"] > 1)).sum())
     assert ncollapsed > 0
+
+
+@pytest.mark.parametrize("kind,n,seed,periodic", CASES[:3])
+def test_parallel_rule_with_refined_top_nodes(kind, n, seed, periodic):
+    """Top nodes at levels 2..6 (NTarget > 1 decompositions)."""
+    from topnodes import refined_top_nodes
+    pos, mass = _inputs(kind, n, seed, periodic)
+    s = run_oracle(pos, mass, periodic)
+    size, origin, _ = s.domain
+    x = s.positions()
+    std2 = oracle.peano_keys(x[:, 0], x[:, 1], x[:, 2], origin, size)
+    rec = refined_top_nodes(std2, nsplit=8)
+    assert rec["Level"].max() > 2
+    s.set_top_nodes(rec)
+    s.tree_build()
+    std = build_rule.key_u128(std2)
+    rev = build_rule.key_u128(s.key)
+    tree = s.tree
+    xl, ml = x.tolist(), s.mass.tolist()
+    for ti, t in enumerate(s.top):
+        if t["Npart"] <= 64:
+            continue
+        base = int(t["Target"])
+        nodes, tp = build_rule.build_topnode(
+            std, rev, xl, ml, int(t["First_Part"]), int(t["Npart"]),
+            t["Pos"].tolist(), ti, size, top_level=int(t["Level"]), base=base)
+        nn = len(nodes["DNext"])
+        sub = tree[base:base + nn]
+        for f in ("DNext", "Bitfield", "DUp", "Npart", "Mass", "CoM"):
+            np.testing.assert_array_equal(sub[f], np.array(nodes[f]),
+                                          err_msg=f)
+        np.testing.assert_array_equal(sub["Pos"][:-1],
+                                      np.array(nodes["Pos"])[:-1])
