@@ -60,6 +60,9 @@ typedef struct {
 	double ewald_accuracy; /* FP32 walk: factor on the opening parameter for
 	                          taking a node's Ewald correction at its centre
 	                          of mass (smaller = more accurate)               */
+	int fp32_order;        /* FP32 walk: multipole order of accepted nodes,
+	                          1 = monopole (the reference's), 2 (or 0) =
+	                          monopole + quadrupole                           */
 } tdv_params;
 
 enum { TDV_WALK_FP64 = 0, TDV_WALK_FP32 = 1 };
@@ -159,10 +162,12 @@ int tdv_set_ewald_table(tdv_context *ctx, const double *table4);
 int tdv_get_domain(tdv_context *ctx, double *size, double origin[3],
 		double center[3]);
 
-/* D[].TNode (domain.h:40-52), 64-byte records in the reference layout. */
+/* D[].TNode (domain.h:40-52), 64-byte records in the reference layout;
+ * out = NULL only reports ntop. */
 int tdv_get_top_nodes(tdv_context *ctx, void *out, int max, int *ntop);
 
-/* Tree and NNodes (Gravity/tree.h:23-39), 96-byte Tree_Node records. */
+/* Tree and NNodes (Gravity/tree.h:23-39), 96-byte Tree_Node records;
+ * out = NULL only reports nnodes. */
 int tdv_get_tree(tdv_context *ctx, void *out, size_t max, size_t *nnodes);
 
 /* Wall times (ms, HIP events on the context's stream) of the stages of the
@@ -174,6 +179,35 @@ int tdv_stage_times(tdv_context *ctx, float ms[8]);
  * the tree walk (interact, Gravity/tree_accel.c:391) and the Ewald walk
  * (interact_with_ewald_cube, Gravity/tree_periodic.c:437). */
 int tdv_interaction_counts(tdv_context *ctx, uint64_t counts[2]);
+
+/* P.Key as 2n uint64 words: the standard keys of the last sort
+ * (reversed = 0, what Sort_Particles_By_Peano_Key leaves) or the reversed
+ * keys (reversed = 1, what Reverse_Peano_Keys leaves, peano.c:84-97). */
+int tdv_get_keys(tdv_context *ctx, uint64_t *keys, int reversed);
+
+/* reorder_collisionless_particles (peano.c:50-82) for P columns the engine
+ * does not hold: applies the last sort's permutation to ncols host columns
+ * of n elements of elem_bytes[c] (4, 8 or 16) bytes each, in place. */
+int tdv_permute_host_columns(tdv_context *ctx, int ncols, void **cols,
+		const int *elem_bytes);
+
+/* Positions, masses, accelerations and Last_Acc_Mag the host changed
+ * since the last sort (Drift_To_Sync_Point, drift.c; Kick_*_Halfstep,
+ * kick.c), in the current particle order; the keys, top nodes and tree
+ * stay as they are (a step without Sig.Tree_Update, accel.c:53-54).
+ * Any pointer but pos may be NULL (column left as it is). */
+int tdv_update_particles(tdv_context *ctx, const double *pos[3],
+		const double *mass, const double *acc[3], const double *last_acc_mag);
+
+/* Active_Particle_List / NActive_Particles (timestep.c:280-299). */
+int tdv_set_active(tdv_context *ctx, int nactive, const int *list);
+
+/* Tree[].CoM/Mass and D[].TNode.CoM/Mass after the host moved them
+ * (Gravity_Tree_Update_Kicks / _Drift, Gravity/tree_update.c:12-104), so a
+ * walk without a rebuild sees the drifted monopoles.  tree = nnodes 96-byte
+ * Tree_Node records, top = ntop 64-byte top-node records. */
+int tdv_set_tree_moments(tdv_context *ctx, const void *tree, size_t nnodes,
+		const void *top, int ntop);
 
 /* Number of particles currently on the device. */
 size_t tdv_num_particles(const tdv_context *ctx);

@@ -13,6 +13,15 @@
 
 #define TDV_EBIT_MASK (TDV_EBIT_DUP | TDV_EBIT_DEPTH | TDV_EBIT_RANGE | TDV_EBIT_TIES)
 
+/* Tree_Node layout, Gravity/tree.h:23-32 (Float = double) */
+struct NodeRecord {
+	int DNext;
+	uint32_t Bitfield;
+	int DUp, Npart;
+	double Pos[3], Mass, CoM[3], Dp[3];
+};
+static_assert(sizeof(NodeRecord) == 96, "Tree_Node is 96 bytes");
+
 /* Top_Tree_Node layout, domain.h:40-52 (records in and out) */
 struct TopRecordIn {
 	uint64_t Key;
@@ -241,6 +250,8 @@ int tdv_create(const tdv_params *p, tdv_context **out)
 	if (p->precision == TDV_WALK_FP32 && p->periodic
 			&& !(p->ewald_size > 0 && p->ewald_accuracy > 0))
 		return TDV_ERR_ARG;
+	if (p->fp32_order < 0 || p->fp32_order > 2)
+		return TDV_ERR_ARG;
 	tdv_context *c = new (std::nothrow) tdv_context();
 	if (!c)
 		return TDV_ERR_NOMEM;
@@ -291,6 +302,8 @@ void tdv_destroy(tdv_context *c)
 	dfree(c->p32);
 	dfree(c->a32);
 	dfree(c->b32);
+	dfree(c->q32);
+	dfree(c->tq32);
 	dfree(c->top32);
 	dfree(c->d_err);
 	dfree(c->d_count);
@@ -804,10 +817,12 @@ struct TopRecord {
 
 int tdv_get_top_nodes(tdv_context *c, void *out, int max, int *ntop)
 {
-	if (!c || !out || !ntop || !c->have_top)
+	if (!c || !ntop || !c->have_top)
 		return tdv_fail(c, TDV_ERR_ARG, "no top nodes yet");
 	const int n = c->ntop;
 	*ntop = n;
+	if (!out)
+		return 0;
 	if (n > max)
 		return tdv_fail(c, TDV_ERR_ARG, "top node buffer too small");
 	std::vector<int> first(n), npart(n), level(n), target(n);
@@ -846,13 +861,6 @@ int tdv_get_top_nodes(tdv_context *c, void *out, int max, int *ntop)
 }
 
 /* Tree_Node layout, Gravity/tree.h:23-32 (Float = double) */
-struct NodeRecord {
-	int DNext;
-	uint32_t Bitfield;
-	int DUp, Npart;
-	double Pos[3], Mass, CoM[3], Dp[3];
-};
-
 int tdv_get_tree(tdv_context *c, void *out, size_t max, size_t *nnodes)
 {
 	if (!c || !nnodes || !c->have_tree)
@@ -895,6 +903,115 @@ int tdv_get_tree(tdv_context *c, void *out, size_t max, size_t *nnodes)
 		rec[j] = r;
 	}
 	return 0;
+}
+
+int tdv_get_keys(tdv_context *c, uint64_t *keys, int reversed)
+{
+	if (!c || !keys || !c->have_domain)
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_get_keys: no sorted keys");
+	const size_t n = c->n;
+	const uint64_t *src = c->key;
+	if (reversed) {
+		tdv_launch_reverse_keys(c, n, c->key, c->key_tmp);
+		src = c->key_tmp;
+	}
+	TDV_HIP(c, hipMemcpyAsync(keys, src, 16 * n, hipMemcpyDeviceToHost,
+			c->stream));
+	return tdv_check(c, hipStreamSynchronize(c->stream), "tdv_get_keys");
+}
+
+int tdv_permute_host_columns(tdv_context *c, int ncols, void **cols,
+		const int *elem_bytes)
+{
+	if (!c || ncols < 0 || (ncols && (!cols || !elem_bytes)) || !c->have_domain)
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_permute_host_columns: no sort");
+	const size_t n = c->n;
+	for (int k = 0; k < ncols; k++)
+		if (!cols[k] || (elem_bytes[k] != 4 && elem_bytes[k] != 8))
+			return tdv_fail(c, TDV_ERR_ARG, "tdv_permute_host_columns: columns "
+					"of 4 or 8 bytes (P.Key: tdv_get_keys)");
+	for (int k = 0; k < ncols; k++) {
+		const size_t b = elem_bytes[k];
+		/* key_tmp holds 16 B per particle, tmp_f64 8 B */
+		TDV_HIP(c, hipMemcpyAsync(c->key_tmp, cols[k], n * b,
+				hipMemcpyHostToDevice, c->stream));
+		if (b == 4)
+			tdv_launch_gather_i32(c, n, c->perm, (const int *)c->key_tmp,
+					(int *)c->tmp_f64);
+		else
+			tdv_launch_gather_f64(c, n, c->perm, (const double *)c->key_tmp,
+					c->tmp_f64);
+		TDV_HIP(c, hipMemcpyAsync(cols[k], c->tmp_f64, n * b,
+				hipMemcpyDeviceToHost, c->stream));
+	}
+	return tdv_check(c, hipStreamSynchronize(c->stream), "permute columns");
+}
+
+int tdv_update_particles(tdv_context *c, const double *pos[3],
+		const double *mass, const double *acc[3], const double *last_acc_mag)
+{
+	if (!c || !pos || !c->n)
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_update_particles: no particles");
+	const size_t b = c->n * sizeof(double);
+	for (int d = 0; d < 3; d++) {
+		if (!pos[d])
+			return tdv_fail(c, TDV_ERR_ARG, "tdv_update_particles: pos");
+		TDV_HIP(c, hipMemcpyAsync(c->pos[d], pos[d], b, hipMemcpyHostToDevice,
+				c->stream));
+		if (acc && acc[d])
+			TDV_HIP(c, hipMemcpyAsync(c->acc[d], acc[d], b,
+					hipMemcpyHostToDevice, c->stream));
+	}
+	if (mass)
+		TDV_HIP(c, hipMemcpyAsync(c->mass, mass, b, hipMemcpyHostToDevice,
+				c->stream));
+	if (last_acc_mag)
+		TDV_HIP(c, hipMemcpyAsync(c->last_acc, last_acc_mag, b,
+				hipMemcpyHostToDevice, c->stream));
+	tdv_launch_pack_pm(c);
+	return tdv_check(c, hipStreamSynchronize(c->stream), "update_particles");
+}
+
+int tdv_set_active(tdv_context *c, int nactive, const int *list)
+{
+	if (!c || nactive < 0 || (size_t)nactive > c->n || (nactive && !list))
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_set_active: bad arguments");
+	for (int i = 0; i < nactive; i++)
+		if (list[i] < 0 || (size_t)list[i] >= c->n)
+			return tdv_fail(c, TDV_ERR_ARG, "tdv_set_active: index out of range");
+	if (nactive)
+		TDV_HIP(c, hipMemcpyAsync(c->active, list, nactive * sizeof(int),
+				hipMemcpyHostToDevice, c->stream));
+	c->nactive = nactive;
+	return tdv_check(c, hipStreamSynchronize(c->stream), "tdv_set_active");
+}
+
+int tdv_set_tree_moments(tdv_context *c, const void *tree, size_t nnodes,
+		const void *top, int ntop)
+{
+	if (!c || !c->have_tree || nnodes != c->nnodes || ntop != c->ntop
+			|| (nnodes && !tree) || !top)
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_set_tree_moments: shape mismatch");
+	const NodeRecord *nr = static_cast<const NodeRecord *>(tree);
+	std::vector<double4> a(nnodes);
+	for (size_t j = 0; j < nnodes; j++)
+		a[j] = make_double4(nr[j].CoM[0], nr[j].CoM[1], nr[j].CoM[2],
+				nr[j].Mass);
+	const TopRecordIn *tr = static_cast<const TopRecordIn *>(top);
+	std::vector<float> tm(ntop), tc(3 * (size_t)ntop);
+	for (int t = 0; t < ntop; t++) {
+		tm[t] = tr[t].Mass;
+		for (int d = 0; d < 3; d++)
+			tc[3 * t + d] = tr[t].CoM[d];
+	}
+	if (nnodes)
+		TDV_HIP(c, hipMemcpyAsync(c->node_a, a.data(), nnodes * 32,
+				hipMemcpyHostToDevice, c->stream));
+	TDV_HIP(c, hipMemcpyAsync(c->top.mass, tm.data(), ntop * 4,
+			hipMemcpyHostToDevice, c->stream));
+	TDV_HIP(c, hipMemcpyAsync(c->top.com, tc.data(), ntop * 12,
+			hipMemcpyHostToDevice, c->stream));
+	return tdv_check(c, hipStreamSynchronize(c->stream), "set_tree_moments");
 }
 
 int tdv_interaction_counts(tdv_context *c, uint64_t counts[2])

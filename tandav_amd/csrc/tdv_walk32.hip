@@ -40,6 +40,7 @@ struct Walk32Params {
 	float dom_f;                     /* domain size in fixed units */
 	float u_scale;                   /* fixed units -> table grid units */
 	int carry_every;
+	int quad;                        /* accepted nodes add their quadrupole */
 };
 
 /* ---- preparation: fixed-point particle / node records -------------------- */
@@ -97,6 +98,75 @@ __global__ void k_prep32_ewald(const double *__restrict__ raw,
 		t32[i] = (float)(raw[i] * scale);
 }
 
+/* Traceless second moment T = sum m (3 y y^T - |y|^2 I), y = x - CoM, of
+ * the particles [first, first + npart) of every internal node (and of every
+ * top node), in FP64 about the FP64 centre of mass, one wavefront per node,
+ * stored scaled (G S^2 masses, fixed lengths) as {Txx, Tyy, Txy, Txz},
+ * {Tyz, 0, 0, 0}; Tzz = -Txx - Tyy. */
+__device__ __forceinline__ void quad_moments(const double4 *__restrict__ pm,
+		int first, int npart, double cx, double cy, double cz, double scale,
+		float4 *__restrict__ out, int lane)
+{
+	double sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0;
+	for (int p = first + lane; p < first + npart; p += 64) {
+		const double4 q = pm[p];
+		const double dx = q.x - cx, dy = q.y - cy, dz = q.z - cz;
+		const double mx = q.w * dx, my = q.w * dy;
+		sxx = fma(mx, dx, sxx);
+		syy = fma(my, dy, syy);
+		szz = fma(q.w * dz, dz, szz);
+		sxy = fma(mx, dy, sxy);
+		sxz = fma(mx, dz, sxz);
+		syz = fma(my, dz, syz);
+	}
+	for (int d = 32; d >= 1; d >>= 1) {
+		sxx += __shfl_xor(sxx, d, 64);
+		syy += __shfl_xor(syy, d, 64);
+		szz += __shfl_xor(szz, d, 64);
+		sxy += __shfl_xor(sxy, d, 64);
+		sxz += __shfl_xor(sxz, d, 64);
+		syz += __shfl_xor(syz, d, 64);
+	}
+	if (lane == 0) {
+		const double tr = sxx + syy + szz;
+		out[0] = make_float4((float)(scale * (3 * sxx - tr)),
+				(float)(scale * (3 * syy - tr)), (float)(scale * 3 * sxy),
+				(float)(scale * 3 * sxz));
+		out[1] = make_float4((float)(scale * 3 * syz), 0.f, 0.f, 0.f);
+	}
+}
+
+__global__ void __launch_bounds__(256) k_prep32_quad(int nn,
+		const int4 *__restrict__ nc, const double4 *__restrict__ na,
+		const double4 *__restrict__ pm, double scale, float4 *__restrict__ q)
+{
+	const int lane = threadIdx.x & 63;
+	const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+	if (j >= nn)
+		return;
+	const int4 c = nc[j];
+	if (c.x <= 0) {                  /* bundles and terminators: never used */
+		if (lane < 2)
+			q[2 * j + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+		return;
+	}
+	const double4 a = na[j];
+	quad_moments(pm, c.w, c.y, a.x, a.y, a.z, scale, q + 2 * j, lane);
+}
+
+__global__ void __launch_bounds__(256) k_prep32_top_quad(int ntop,
+		const TopNodes T, const double4 *__restrict__ pm, double scale,
+		float4 *__restrict__ q)
+{
+	const int lane = threadIdx.x & 63;
+	const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+	if (t >= ntop)
+		return;
+	quad_moments(pm, T.first[t], T.npart[t], (double)T.com[3 * t],
+			(double)T.com[3 * t + 1], (double)T.com[3 * t + 2], scale,
+			q + 2 * t, lane);
+}
+
 /* ---- the walk ---------------------------------------------------------- */
 
 struct Acc32 {
@@ -138,6 +208,30 @@ __device__ __forceinline__ void newton32(Acc32 &a, float m, float dx,
 	a.y = fmaf(fac, dy, a.y);
 	a.z = fmaf(fac, dz, a.z);
 	a.cost++;
+}
+
+/* Quadrupole term of a node seen at d (particle -> node CoM):
+ * a = G [-T d / r^5 + 5/2 (d.T d) d / r^7], written with u = d / r so that
+ * no intermediate leaves the FP32 range: a = r^-4 (5/2 (u.T u) u - T u).
+ * Skipped inside the softening length, where the expansion does not hold. */
+__device__ __forceinline__ void quad32(Acc32 &a, const float4 qa, float qyz,
+		float dx, float dy, float dz, const Walk32Params &w)
+{
+	const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+	if (r2 < w.eps2 || r2 == 0.f)
+		return;
+	const float ri = __frsqrt_rn(r2);
+	const float ux = dx * ri, uy = dy * ri, uz = dz * ri;
+	const float txx = qa.x, tyy = qa.y, tzz = -(qa.x + qa.y), txy = qa.z,
+			txz = qa.w, tyz = qyz;
+	const float tux = fmaf(txx, ux, fmaf(txy, uy, txz * uz));
+	const float tuy = fmaf(txy, ux, fmaf(tyy, uy, tyz * uz));
+	const float tuz = fmaf(txz, ux, fmaf(tyz, uy, tzz * uz));
+	const float c = 2.5f * fmaf(ux, tux, fmaf(uy, tuy, uz * tuz));
+	const float ri2 = ri * ri, ri4 = ri2 * ri2;
+	a.x = fmaf(ri4, fmaf(c, ux, -tux), a.x);
+	a.y = fmaf(ri4, fmaf(c, uy, -tuy), a.y);
+	a.z = fmaf(ri4, fmaf(c, uz, -tuz), a.z);
 }
 
 /* Ewald_Correction (Gravity/periodic.c:35-113) on the FP32 table, already
@@ -227,7 +321,8 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 		double *__restrict__ az, double *__restrict__ gx,
 		double *__restrict__ gy, double *__restrict__ gz,
 		double *__restrict__ cost, unsigned long long *counter,
-		const int4 *__restrict__ top32)
+		const int4 *__restrict__ top32, const float4 *__restrict__ q32,
+		const float4 *__restrict__ tq32)
 {
 	const int gid = blockIdx.x * 64 + threadIdx.x;
 	const bool live = gid < nact;
@@ -252,6 +347,8 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 		want = want || r2 < 3.f * ts * ts;
 		if (!__any(want)) {
 			newton32(a, tm, rx, ry, rz, w);
+			if (w.quad)
+				quad32(a, tq32[2 * j], tq32[2 * j + 1].x, rx, ry, rz, w);
 			if (w.periodic)
 				ewald32(a, tab, tm, rx, ry, rz, w);
 			tick(a, every);
@@ -329,6 +426,8 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 			}
 			if (!__any(open)) {
 				newton32(a, m, rx, ry, rz, w);
+				if (w.quad)
+					quad32(a, q32[2 * node], q32[2 * node + 1].x, rx, ry, rz, w);
 				if (pending)
 					ewald32(a, tab, m, rx, ry, rz, w);
 				tick(a, every);
@@ -385,17 +484,43 @@ int tdv_walk32(tdv_context *c, int use_bh)
 			(void)hipFree(c->a32);
 		if (c->b32)
 			(void)hipFree(c->b32);
+		if (c->q32)
+			(void)hipFree(c->q32);
+		c->a32 = c->b32 = nullptr;
+		c->q32 = nullptr;
+		c->n32_cap = 0;
 		TDV_HIP(c, hipMalloc((void **)&c->a32, c->node_cap * sizeof(int4)));
 		TDV_HIP(c, hipMalloc((void **)&c->b32, c->node_cap * sizeof(int4)));
+		TDV_HIP(c, hipMalloc((void **)&c->q32,
+				2 * c->node_cap * sizeof(float4)));
 		c->n32_cap = c->node_cap;
 	}
+	const bool quad = c->par.fp32_order != 1;
 	hipLaunchKernelGGL(k_prep32_particles, dim3(tdv_blocks(n, 256)), dim3(256),
 			0, c->stream, n, c->pm, o[0], o[1], o[2], S, mscale, c->p32);
 	if (c->top32_cap < c->top_cap) {
 		if (c->top32)
 			(void)hipFree(c->top32);
+		if (c->tq32)
+			(void)hipFree(c->tq32);
+		c->top32 = nullptr;
+		c->tq32 = nullptr;
+		c->top32_cap = 0;
 		TDV_HIP(c, hipMalloc((void **)&c->top32, c->top_cap * sizeof(int4)));
+		TDV_HIP(c, hipMalloc((void **)&c->tq32,
+				2 * (size_t)c->top_cap * sizeof(float4)));
 		c->top32_cap = c->top_cap;
+	}
+	/* quadrupoles: masses G S^2, lengths S -> scale G S^4 */
+	const double qscale = mscale * S * S;
+	if (quad) {
+		hipLaunchKernelGGL(k_prep32_top_quad,
+				dim3(tdv_blocks((size_t)c->ntop, 4)), dim3(256), 0, c->stream,
+				c->ntop, c->top, c->pm, qscale, c->tq32);
+		if (nn)
+			hipLaunchKernelGGL(k_prep32_quad, dim3(tdv_blocks(nn, 4)),
+					dim3(256), 0, c->stream, (int)nn, c->node_c, c->node_a,
+					c->pm, qscale, c->q32);
 	}
 	hipLaunchKernelGGL(k_prep32_top, dim3(tdv_blocks((size_t)c->ntop, 64)),
 			dim3(64), 0, c->stream, c->ntop, c->top, o[0], o[1], o[2], S,
@@ -430,10 +555,11 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	w.ewald_acc = (float)c->par.ewald_accuracy;
 	w.u_scale = (float)(2.0 * TDV_NE / 4294967296.0);
 	w.carry_every = 64;
+	w.quad = quad ? 1 : 0;
 	hipLaunchKernelGGL(k_walk32, dim3(tdv_blocks((size_t)c->nactive, 64)),
 			dim3(64), 0, c->stream, c->nactive, c->active, c->p32, c->last_acc,
 			c->top, c->a32, c->b32, c->node_c, c->ewald32, w, c->acc[0],
 			c->acc[1], c->acc[2], c->gacc[0], c->gacc[1], c->gacc[2], c->cost,
-			c->d_count, c->top32);
+			c->d_count, c->top32, c->q32, c->tq32);
 	return tdv_check(c, hipGetLastError(), "k_walk32");
 }

@@ -35,7 +35,8 @@ class TdvParams(ctypes.Structure):
                 ("vector_size", ctypes.c_int), ("compat", ctypes.c_int),
                 ("device", ctypes.c_int), ("precision", ctypes.c_int),
                 ("ewald_size", ctypes.c_double),
-                ("ewald_accuracy", ctypes.c_double)]
+                ("ewald_accuracy", ctypes.c_double),
+                ("fp32_order", ctypes.c_int)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -90,6 +91,14 @@ def load_library(path=LIB_PATH):
     L.tdv_stage_times.argtypes = [_ctx, ctypes.POINTER(ctypes.c_float)]
     L.tdv_interaction_counts.argtypes = [_ctx,
                                          ctypes.POINTER(ctypes.c_uint64)]
+    L.tdv_get_keys.argtypes = [_ctx, _vp, ctypes.c_int]
+    L.tdv_permute_host_columns.argtypes = [_ctx, ctypes.c_int,
+                                           ctypes.POINTER(_vp),
+                                           ctypes.POINTER(ctypes.c_int)]
+    L.tdv_update_particles.argtypes = [_ctx, ctypes.POINTER(_dp), _dp,
+                                       ctypes.POINTER(_dp), _dp]
+    L.tdv_set_active.argtypes = [_ctx, ctypes.c_int, _vp]
+    L.tdv_set_tree_moments.argtypes = [_ctx, _vp, _sz, _vp, ctypes.c_int]
     L.tdv_num_particles.argtypes = [_ctx]
     L.tdv_num_particles.restype = _sz
     _lib = L
@@ -113,14 +122,15 @@ class Engine:
     def __init__(self, periodic=False, boxsize=1.0, gravity=1.0,
                  softening=0.01, theta_bh=0.1, alpha_rel=0.02, vector_size=64,
                  compat=True, device=0, ewald_table=None, precision="fp64",
-                 ewald_size=0.125, ewald_accuracy=0.1):
+                 ewald_size=0.125, ewald_accuracy=0.1, fp32_order=2):
         self.lib = load_library()
         prec = {"fp64": 0, "fp32": 1}[precision]
         self.params = TdvParams(int(periodic), float(boxsize), float(gravity),
                                 float(softening), float(theta_bh),
                                 float(alpha_rel), int(vector_size),
                                 int(compat), int(device), prec,
-                                float(ewald_size), float(ewald_accuracy))
+                                float(ewald_size), float(ewald_accuracy),
+                                int(fp32_order))
         self._c = _ctx()
         rc = self.lib.tdv_create(ctypes.byref(self.params),
                                  ctypes.byref(self._c))
@@ -271,6 +281,60 @@ class Engine:
                  "tdv_forcetest")
         return out
 
+    def keys(self, reversed_=True):
+        """P.Key of the last sort: standard (reversed_=False) or reversed
+        (Reverse_Peano_Keys, peano.c:84-97), (n, 2) uint64 words (lo, hi)."""
+        out = np.zeros((self.n, 2), np.uint64)
+        self._ok(self.lib.tdv_get_keys(self._c, out.ctypes.data_as(_vp),
+                                       int(reversed_)), "tdv_get_keys")
+        return out
+
+    def permute_columns(self, *cols):
+        """Apply the last sort's permutation to host columns (4- or 8-byte
+        elements) in place (reorder_collisionless_particles, peano.c:50-82)."""
+        cs = [np.ascontiguousarray(c) for c in cols]
+        for c, orig in zip(cs, cols):
+            if c is not orig or c.itemsize not in (4, 8) or c.size != self.n:
+                raise TDVError("permute_columns: contiguous 4/8-byte columns "
+                               "of n elements")
+        ptrs = (_vp * len(cs))(*[c.ctypes.data_as(_vp) for c in cs])
+        sizes = (ctypes.c_int * len(cs))(*[c.itemsize for c in cs])
+        self._ok(self.lib.tdv_permute_host_columns(self._c, len(cs), ptrs,
+                                                   sizes),
+                 "tdv_permute_host_columns")
+
+    def update_particles(self, pos, mass=None, acc=None, last_acc_mag=None):
+        """Drifted positions (and masses, accelerations, |a_prev|) in the
+        current particle order; keys, top nodes and tree are kept."""
+        p = [np.ascontiguousarray(pos[:, d], dtype=np.float64)
+             for d in range(3)]
+        a = None if acc is None else [
+            np.ascontiguousarray(acc[:, d], dtype=np.float64)
+            for d in range(3)]
+        m = None if mass is None else np.ascontiguousarray(mass, np.float64)
+        la = (None if last_acc_mag is None
+              else np.ascontiguousarray(last_acc_mag, dtype=np.float64))
+        self._ok(self.lib.tdv_update_particles(
+            self._c, _col3(p), _d(m) if m is not None else None,
+            _col3(a) if a else None, _d(la) if la is not None else None),
+            "tdv_update_particles")
+
+    def set_active(self, idx):
+        """Active_Particle_List (timestep.c:280-299)."""
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        self._ok(self.lib.tdv_set_active(self._c, idx.size,
+                                         idx.ctypes.data_as(_vp)),
+                 "tdv_set_active")
+
+    def set_tree_moments(self, tree, top):
+        """Tree[] / D[].TNode CoM and Mass moved on the host
+        (Gravity/tree_update.c:12-104)."""
+        tree = np.ascontiguousarray(tree, dtype=NODE_DTYPE)
+        top = np.ascontiguousarray(top, dtype=TOP_DTYPE)
+        self._ok(self.lib.tdv_set_tree_moments(
+            self._c, tree.ctypes.data_as(_vp), len(tree),
+            top.ctypes.data_as(_vp), len(top)), "tdv_set_tree_moments")
+
     # -- element-wise functions -------------------------------------------------
     def peano_keys(self, x, y, z, origin, size, reversed_=False):
         """Peano_Key / Reversed_Peano_Key (peano.c:122-276)."""
@@ -323,12 +387,14 @@ class Engine:
         return float(s[0]), o, c
 
     def top_nodes(self):
-        buf = np.zeros(64, dtype=TOP_DTYPE)
         n = ctypes.c_int()
-        self._ok(self.lib.tdv_get_top_nodes(self._c, buf.ctypes.data_as(_vp),
-                                            64, ctypes.byref(n)),
+        self._ok(self.lib.tdv_get_top_nodes(self._c, None, 0, ctypes.byref(n)),
                  "tdv_get_top_nodes")
-        return buf[:n.value].copy()
+        buf = np.zeros(n.value, dtype=TOP_DTYPE)
+        self._ok(self.lib.tdv_get_top_nodes(self._c, buf.ctypes.data_as(_vp),
+                                            n.value, ctypes.byref(n)),
+                 "tdv_get_top_nodes")
+        return buf
 
     def tree(self):
         n = _sz()

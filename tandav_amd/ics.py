@@ -60,7 +60,8 @@ def _zeldovich_displacement(ngrid, seed, slope, rms_frac):
         inv_k2 = np.where(k2 > 0, 1.0 / k2, 0.0)
     psi = []
     for kd in (kx, ky, kzz):
-        comp = np.fft.irfftn(1j * kd * inv_k2 * dk, s=(ngrid,) * 3)
+        comp = np.fft.irfftn(1j * kd * inv_k2 * dk, s=(ngrid,) * 3,
+                             axes=(0, 1, 2))
         psi.append(comp.reshape(-1))
     psi = np.stack(psi, 1)
     rms = np.sqrt(np.mean(np.sum(psi * psi, axis=1)))

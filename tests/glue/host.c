@@ -1,0 +1,310 @@
+/*
+ * host.c -- a minimal host program around the reference-side binding
+ * (tandav_amd/glue/tandav_glue.c): it owns the particle columns, the top
+ * nodes and the signals the way Tandav does, and calls the binding's team
+ * functions from an OpenMP team, in the order of Tandav's main loop
+ * (main.c:37-95, domain.c:67-145, accel.c:15-71, kick.c, drift.c).  The
+ * tests drive it through ctypes.
+ */
+#include "host_env.h"
+
+struct Particle_Data P;
+union Domain_Node_List * restrict D = NULL;
+uint32_t NTop_Nodes = 0;
+struct Domain_Properties Domain;
+double Epsilon[NPARTYPE], Epsilon2[NPARTYPE], Epsilon3[NPARTYPE];
+uint32_t NNodes = 0;
+struct Simulation_Signals Sig;
+struct Local_Task_Properties Task;
+struct Global_Simulation_Properties Sim;
+struct Parameters_From_File Param;
+struct Constants_In_Code_Units Const;
+int * restrict Active_Particle_List = NULL;
+int NActive_Particles = 0;
+double Host_Theta_BH = 0.1, Host_Alpha_Rel = 0.005;
+
+static int Threads = 4;
+static int Max_Active_Bin = 0;
+static size_t N = 0;
+
+/* Make_Active_Particle_List (timestep.c:280-299) */
+void Make_Active_Particle_List(void)
+{
+	#pragma omp single
+	{
+	int k = 0;
+	for (int i = 0; i < Task.Npart_Total; i++)
+		if (P.Time_Bin[i] <= Max_Active_Bin)
+			Active_Particle_List[k++] = i;
+	NActive_Particles = k;
+	} // omp single
+}
+
+static void *column(size_t n, size_t bytes)
+{
+	void *p = calloc(n ? n : 1, bytes);
+	if (!p) {
+		fprintf(stderr, "host: out of memory\n");
+		abort();
+	}
+	return p;
+}
+
+static void set_task(void)
+{
+	#pragma omp parallel num_threads(Threads)
+	{
+	Task.Npart_Total = (int)N;
+	Task.Is_Master = false;
+	Task.Thread_ID = omp_get_thread_num();
+	Task.Seed[0] = 11;
+	Task.Seed[1] = 22;
+	Task.Seed[2] = 33;
+	memset(&Sig, 0, sizeof(Sig));
+	}
+}
+
+/* Read_and_Init + Setup (main.c:22-25): the particle columns, the run's
+ * parameters, Setup_Gravity_Tree (setup.c:21) and Gravity_Periodic_Init
+ * (init.c:71). */
+int host_init(int n, const double *x, const double *y, const double *z,
+		const double *vx, const double *vy, const double *vz,
+		const double *m, const int *time_bin, double boxsize, double gravity,
+		double softening, double theta_bh, double alpha_rel, int threads)
+{
+	N = (size_t)n;
+	Threads = threads > 0 ? threads : 1;
+	P.Type = column(N, sizeof(int));
+	P.Time_Bin = column(N, sizeof(int));
+	P.It_Drift_Pos = column(N, sizeof(intime_t));
+	P.It_Kick_Pos = column(N, sizeof(intime_t));
+	P.Key = column(N, sizeof(peanoKey));
+	P.ID = column(N, sizeof(ID_t));
+	P.Cost = column(N, sizeof(Float));
+	P.Mass = column(N, sizeof(Float));
+	P.Last_Acc_Mag = column(N, sizeof(Float));
+	P.Tree_Parent = column(N, sizeof(Float));
+	for (int d = 0; d < 3; d++) {
+		P.Pos[d] = column(N, sizeof(Float));
+		P.Vel[d] = column(N, sizeof(Float));
+		P.Acc[d] = column(N, sizeof(Float));
+		P.Grav_Acc[d] = column(N, sizeof(Float));
+	}
+	Active_Particle_List = column(N, sizeof(int));
+	for (size_t i = 0; i < N; i++) {
+		P.Pos[0][i] = x[i];
+		P.Pos[1][i] = y[i];
+		P.Pos[2][i] = z[i];
+		P.Vel[0][i] = vx[i];
+		P.Vel[1][i] = vy[i];
+		P.Vel[2][i] = vz[i];
+		P.Mass[i] = m[i];
+		P.ID[i] = (ID_t)i;
+		P.Type[i] = 1;
+		P.Time_Bin[i] = time_bin ? time_bin[i] : 0;
+		P.It_Drift_Pos[i] = (intime_t)(3 * i);
+		P.It_Kick_Pos[i] = (intime_t)(5 * i);
+	}
+	Sim.Npart_Total = N;
+	Sim.Boxsize[0] = Sim.Boxsize[1] = Sim.Boxsize[2] = boxsize;
+	for (int t = 0; t < NPARTYPE; t++)
+		Param.Grav_Softening[t] = softening;
+	Const.Gravity = gravity;
+	Host_Theta_BH = theta_bh;
+	Host_Alpha_Rel = alpha_rel;
+	set_task();
+	Setup_Gravity_Tree();
+#ifdef PERIODIC
+	Gravity_Periodic_Init();
+#endif
+	return 0;
+}
+
+void host_set_domain(double size, const double origin[3])
+{
+	Domain.Size = size;
+	for (int d = 0; d < 3; d++)
+		Domain.Origin[d] = origin[d];
+}
+
+void host_set_max_active_bin(int bin)
+{
+	Max_Active_Bin = bin;
+}
+
+void host_make_active(void)
+{
+	#pragma omp parallel num_threads(Threads)
+	Make_Active_Particle_List();
+}
+
+/* Domain_Decomposition, first half (domain.c:67-73) */
+void host_sort(void)
+{
+	#pragma omp parallel num_threads(Threads)
+	Sort_Particles_By_Peano_Key();
+}
+
+/* transform_bunches_into_top_nodes (domain.c:220-250): the test hands in
+ * the top-node records */
+int host_set_top_nodes(const void *records, int ntop)
+{
+	D = realloc(D, (size_t)ntop * sizeof(*D));
+	if (!D)
+		return 1;
+	memcpy(D, records, (size_t)ntop * sizeof(*D));
+	NTop_Nodes = (uint32_t)ntop;
+	return 0;
+}
+
+/* Domain_Decomposition, second half (domain.c:134-140) */
+void host_finish_decomposition(void)
+{
+	#pragma omp parallel num_threads(Threads)
+	{
+	Sort_Particles_By_Peano_Key();
+	Reverse_Peano_Keys();
+	Sig.Tree_Update = true;
+	}
+}
+
+/* Update(BEFORE_MAIN_LOOP) / the main loop's Compute_Acceleration
+ * (update.c:15-21, main.c:80) */
+void host_compute_acceleration(int prepare_step)
+{
+	#pragma omp parallel num_threads(Threads)
+	{
+	Sig.Prepare_Step = prepare_step != 0;
+	Compute_Acceleration();
+	Sig.Prepare_Step = false;
+	}
+}
+
+/* Gravity_Tree_Acceleration on its own, BH or relative criterion */
+void host_tree_acceleration(int use_bh)
+{
+	#pragma omp parallel num_threads(Threads)
+	{
+	Sig.Use_BH_Criterion = use_bh != 0;
+	#pragma omp for
+	for (int i = 0; i < NActive_Particles; i++) {
+		const int ip = Active_Particle_List[i];
+		P.Acc[0][ip] = P.Acc[1][ip] = P.Acc[2][ip] = 0;
+	}
+	Gravity_Tree_Acceleration();
+	Sig.Use_BH_Criterion = false;
+	}
+}
+
+/* the tree half of Kick_*_Halfstep (kick.c:32) for the active particles */
+void host_tree_kicks(double dt)
+{
+	#pragma omp parallel num_threads(Threads)
+	{
+	#pragma omp for
+	for (int i = 0; i < NActive_Particles; i++)
+		Gravity_Tree_Update_Kicks(Active_Particle_List[i], dt);
+	}
+}
+
+/* Drift_To_Sync_Point (drift.c:20-45): positions, then the tree */
+void host_drift(double dt)
+{
+	#pragma omp parallel num_threads(Threads)
+	{
+	#pragma omp for
+	for (int i = 0; i < (int)N; i++)
+		for (int d = 0; d < 3; d++)
+			P.Pos[d][i] += dt * P.Vel[d][i];
+	Gravity_Tree_Update_Drift(dt);
+	}
+}
+
+void host_tree_free(void)
+{
+	#pragma omp parallel num_threads(Threads)
+	Gravity_Tree_Free();
+}
+
+#ifdef PERIODIC
+void host_ewald_correction(int n, const double *dr, double *f)
+{
+	#pragma omp parallel for num_threads(Threads)
+	for (int i = 0; i < n; i++)
+		Ewald_Correction(dr + 3 * i, f + 3 * i);
+}
+#endif
+
+void host_peano_keys(int n, const double *x, const double *y,
+		const double *z, uint64_t *std2, uint64_t *rev2, uint64_t *short1,
+		uint64_t *rshort1)
+{
+	#pragma omp parallel for num_threads(Threads)
+	for (int i = 0; i < n; i++) {
+		const peanoKey s = Peano_Key(x[i], y[i], z[i]);
+		const peanoKey r = Reversed_Peano_Key(x[i], y[i], z[i]);
+		std2[2 * i] = (uint64_t)s;
+		std2[2 * i + 1] = (uint64_t)(s >> 64);
+		rev2[2 * i] = (uint64_t)r;
+		rev2[2 * i + 1] = (uint64_t)(r >> 64);
+		short1[i] = Short_Peano_Key(x[i], y[i], z[i]);
+		rshort1[i] = Reversed_Short_Peano_Key(x[i], y[i], z[i]);
+	}
+}
+
+void host_reverse_peano_key(int n, const uint64_t *in2, uint64_t *out2)
+{
+	for (int i = 0; i < n; i++) {
+		const peanoKey k = ((peanoKey)in2[2 * i + 1] << 64) | in2[2 * i];
+		const peanoKey r = Reverse_Peano_Key(k);
+		out2[2 * i] = (uint64_t)r;
+		out2[2 * i + 1] = (uint64_t)(r >> 64);
+	}
+}
+
+int host_cmp_keys(const uint64_t *a2, const uint64_t *b2)
+{
+	const peanoKey a = ((peanoKey)a2[1] << 64) | a2[0];
+	const peanoKey b = ((peanoKey)b2[1] << 64) | b2[0];
+	return cmp_peanoKeys(&a, &b);
+}
+
+/* column access for the tests: name = P field, comp = component */
+void *host_column(const char *name, int comp)
+{
+#define COL(f) if (!strcmp(name, #f)) return (void *)P.f;
+#define COL3(f) if (!strcmp(name, #f)) return (void *)P.f[comp];
+	COL(Type) COL(Time_Bin) COL(It_Drift_Pos) COL(It_Kick_Pos) COL(Key)
+	COL(ID) COL(Cost) COL(Mass) COL(Last_Acc_Mag) COL(Tree_Parent)
+	COL3(Pos) COL3(Vel) COL3(Acc) COL3(Grav_Acc)
+#undef COL
+#undef COL3
+	return NULL;
+}
+
+void *host_tree(uint32_t *nnodes)
+{
+	*nnodes = NNodes;
+	return Tree;
+}
+
+void *host_top_nodes(uint32_t *ntop)
+{
+	*ntop = NTop_Nodes;
+	return D;
+}
+
+int host_nactive(void)
+{
+	return NActive_Particles;
+}
+
+double host_node_size(int node)
+{
+	return Node_Size(node);
+}
+
+int host_level(int node)
+{
+	return Level(node);
+}

@@ -49,12 +49,12 @@ def test_forcetest_matches_cpu_direct_sum(periodic, table):
     assert rel_err(got, want).max() < 1e-11
 
 
-def _fp32_error(pos, m, periodic, soft, alpha, esize, table):
+def _fp32_error(pos, m, periodic, soft, alpha, esize, table, order=2):
     e64 = Engine(periodic=periodic, softening=soft, alpha_rel=alpha,
                  ewald_table=table if periodic else None)
     e32 = Engine(periodic=periodic, softening=soft, alpha_rel=alpha,
                  precision="fp32", ewald_size=esize, compat=False,
-                 ewald_table=table if periodic else None)
+                 ewald_table=table if periodic else None, fp32_order=order)
     out = []
     for e in (e64, e32):
         e.set_particles(pos, m)
@@ -85,3 +85,19 @@ def test_fp32_group_walk_periodic_box(table):
     # reference's own FP64 walk is far less accurate on this box
     assert np.percentile(r32, 99) <= 2e-3
     assert np.percentile(r32, 99) < np.percentile(r64, 99)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_fp32_quadrupoles_reduce_the_error(periodic, table):
+    """The quadrupole term of accepted nodes (fp32_order=2) against the
+    monopole-only walk (fp32_order=1) at the same opening criterion."""
+    if periodic:
+        pos, _, m = ics.zeldovich_box(48, seed=7)
+        args = (True, 1 / 1920., 0.005, 1 / 64., table)
+    else:
+        pos, _, m = ics.hernquist_sphere(65536, seed=1)
+        args = (False, 0.01, 0.005, 0.0625, table)
+    r2, _ = _fp32_error(pos, m, *args, order=2)
+    r1, _ = _fp32_error(pos, m, *args, order=1)
+    assert np.percentile(r2, 99) < 0.7 * np.percentile(r1, 99)
+    assert np.median(r2) < 0.7 * np.median(r1)
