@@ -63,6 +63,11 @@ typedef struct {
 	int fp32_order;        /* FP32 walk: multipole order of accepted nodes,
 	                          1 = monopole (the reference's), 2 (or 0) =
 	                          monopole + quadrupole                           */
+	int fp32_criterion;    /* FP32 walk: relative opening test, 2 (or 0) =
+	                          M s^2 > r^4 |a| alpha / G (the reference's,
+	                          Gravity/tree_accel.c:257), 3 = M s^3 > r^5 |a|
+	                          alpha / G (the error bound of the quadrupole
+	                          walk)                                           */
 } tdv_params;
 
 enum { TDV_WALK_FP64 = 0, TDV_WALK_FP32 = 1 };
@@ -199,7 +204,10 @@ int tdv_permute_host_columns(tdv_context *ctx, int ncols, void **cols,
 int tdv_update_particles(tdv_context *ctx, const double *pos[3],
 		const double *mass, const double *acc[3], const double *last_acc_mag);
 
-/* Active_Particle_List / NActive_Particles (timestep.c:280-299). */
+/* Active_Particle_List / NActive_Particles (timestep.c:280-299): the
+ * particles (indices in the current order) the next walk computes.  A sort
+ * (tdv_sort_particles, tdv_domain_decomposition) resets it to every
+ * particle. */
 int tdv_set_active(tdv_context *ctx, int nactive, const int *list);
 
 /* Tree[].CoM/Mass and D[].TNode.CoM/Mass after the host moved them

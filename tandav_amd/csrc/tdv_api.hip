@@ -252,6 +252,9 @@ int tdv_create(const tdv_params *p, tdv_context **out)
 		return TDV_ERR_ARG;
 	if (p->fp32_order < 0 || p->fp32_order > 2)
 		return TDV_ERR_ARG;
+	if (p->fp32_criterion != 0 && p->fp32_criterion != 2
+			&& p->fp32_criterion != 3)
+		return TDV_ERR_ARG;
 	tdv_context *c = new (std::nothrow) tdv_context();
 	if (!c)
 		return TDV_ERR_NOMEM;
@@ -450,8 +453,17 @@ static int computational_domain(tdv_context *c)
 	return tdv_domain_nonperiodic(c);
 }
 
+__global__ void k_iota(int n, int *__restrict__ a)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		a[i] = i;
+}
+
 /* Sort_Particles_By_Peano_Key (peano.c:20-48) with the current domain:
- * keys, sort, reorder of every column */
+ * keys, sort, reorder of every column.  The active list's indices belong to
+ * the old order, so it is reset to every particle (a sync point, as at every
+ * domain decomposition of the reference); tdv_set_active narrows it. */
 static int keys_sort_reorder(tdv_context *c)
 {
 	const size_t n = c->n;
@@ -483,6 +495,11 @@ static int keys_sort_reorder(tdv_context *c)
 				hipMemcpyDeviceToDevice, c->stream));
 	}
 	tdv_launch_pack_pm(c);
+	if ((size_t)c->nactive != n) {
+		hipLaunchKernelGGL(k_iota, dim3(tdv_blocks(n, 256)), dim3(256), 0,
+				c->stream, (int)n, c->active);
+		c->nactive = (int)n;
+	}
 	(void)hipEventRecord(c->ev[EV_REORDER1], c->stream);
 	TDV_HIP(c, hipGetLastError());
 	TDV_TRY(tdv_check_err_flags(c));

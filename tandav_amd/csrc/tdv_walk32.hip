@@ -41,7 +41,19 @@ struct Walk32Params {
 	float u_scale;                   /* fixed units -> table grid units */
 	int carry_every;
 	int quad;                        /* accepted nodes add their quadrupole */
+	int crit3;                       /* open if M s^3 > r^5 |a| alpha / G */
 };
+
+/* The relative opening test of Gravity/tree_accel.c:257, M s^2 > r^4 crit
+ * (crit = |a_prev| alpha / G in the kernel's units), or its next-order form
+ * M s^3 > r^5 crit, which bounds the octupole error left once quadrupoles
+ * are added.  Written as M s^2 (s/r) so no product leaves the FP32 range. */
+__device__ __forceinline__ bool rel_open(float m, float s, float r2,
+		float crit, const Walk32Params &w)
+{
+	const float lhs = w.crit3 ? m * s * s * (s * __frsqrt_rn(r2)) : m * s * s;
+	return lhs > (r2 * crit) * r2;
+}
 
 /* ---- preparation: fixed-point particle / node records -------------------- */
 
@@ -78,8 +90,11 @@ __global__ void k_prep32_nodes(size_t nn, const double4 *__restrict__ na,
 			(int)fixp(b.z, o2, S), __float_as_int((float)(b.w * S)));
 }
 
+/* top nodes: t32[t] = {CoM, mass'}, tb32[t] = {centre, size} in fixed
+ * units (the float centre of domain.c:220-250) */
 __global__ void k_prep32_top(int ntop, const TopNodes T, double o0, double o1,
-		double o2, double S, double mscale, int4 *__restrict__ t32)
+		double o2, double S, double mscale, double dom_f,
+		int4 *__restrict__ t32, int4 *__restrict__ tb32)
 {
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= ntop)
@@ -88,6 +103,9 @@ __global__ void k_prep32_top(int ntop, const TopNodes T, double o0, double o1,
 	t32[t] = make_int4((int)fixp(T.com[3 * t], o0, S),
 			(int)fixp(T.com[3 * t + 1], o1, S), (int)fixp(T.com[3 * t + 2], o2, S),
 			__float_as_int((float)m));
+	tb32[t] = make_int4((int)fixp(T.pos[3 * t], o0, S),
+			(int)fixp(T.pos[3 * t + 1], o1, S), (int)fixp(T.pos[3 * t + 2], o2, S),
+			__float_as_int((float)(dom_f / (double)(1 << T.level[t]))));
 }
 
 __global__ void k_prep32_ewald(const double *__restrict__ raw,
@@ -321,8 +339,8 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 		double *__restrict__ az, double *__restrict__ gx,
 		double *__restrict__ gy, double *__restrict__ gz,
 		double *__restrict__ cost, unsigned long long *counter,
-		const int4 *__restrict__ top32, const float4 *__restrict__ q32,
-		const float4 *__restrict__ tq32)
+		const int4 *__restrict__ top32, const int4 *__restrict__ topb32,
+		const float4 *__restrict__ q32, const float4 *__restrict__ tq32)
 {
 	const int gid = blockIdx.x * 64 + threadIdx.x;
 	const bool live = gid < nact;
@@ -342,17 +360,24 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 				rz = fdiff(tc.z, me.z);
 		const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
 		bool want = w.use_bh ? (ts * ts > r2 * w.theta_bh)
-				: (tm * ts * ts > (r2 * crit) * r2);
+				: rel_open(tm, ts, r2, crit, w);
 		/* the top node holds the particle (or is close): walk it */
 		want = want || r2 < 3.f * ts * ts;
+		bool newton_top = true;       /* Newton terms still to add below */
 		if (!__any(want)) {
 			newton32(a, tm, rx, ry, rz, w);
 			if (w.quad)
 				quad32(a, tq32[2 * j], tq32[2 * j + 1].x, rx, ry, rz, w);
-			if (w.periodic)
-				ewald32(a, tab, tm, rx, ry, rz, w);
 			tick(a, every);
-			continue;
+			if (!w.periodic)
+				continue;
+			const int4 tb = topb32[j];
+			if (__all(ewald_ok(tm, ts, rx, ry, rz, fdiff(tb.x, me.x),
+					fdiff(tb.y, me.y), fdiff(tb.z, me.z), crit, w))) {
+				ewald32(a, tab, tm, rx, ry, rz, w);
+				continue;
+			}
+			newton_top = false;       /* walk it for the correction only */
 		}
 		if (T.npart[j] <= w.vs) {
 			const int f = T.first[j], l = f + T.npart[j];
@@ -361,7 +386,8 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 				const float dx = fdiff(pq.x, me.x), dy = fdiff(pq.y, me.y),
 						dz = fdiff(pq.z, me.z);
 				const float m = __int_as_float(pq.w);
-				newton32(a, m, dx, dy, dz, w);
+				if (newton_top)
+					newton32(a, m, dx, dy, dz, w);
 				if (w.periodic && (dx != 0.f || dy != 0.f || dz != 0.f))
 					ewald32(a, tab, m, dx, dy, dz, w);
 				tick(a, every);
@@ -371,11 +397,17 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 		const int start = T.target[j];
 		int node = start;
 		int ew_until = w.periodic ? -1 : 0x7FFFFFFF;  /* correction done below */
+		int nw_until = newton_top ? -1 : 0x7FFFFFFF;  /* Newton done below */
 		for (;;) {
 			const int4 c = nc[node];
 			if (c.x == 0 && node != start)
 				break;
 			const bool pending = node >= ew_until;
+			const bool ndone = node < nw_until;
+			if (ndone && !pending) {      /* nothing left in this subtree */
+				node += c.x > 0 ? c.x : 1;
+				continue;
+			}
 			if (c.x < 0) {
 				const int f = -c.x - 1, l = f + c.y;
 				bool per_particle = false;
@@ -395,16 +427,19 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 					else
 						per_particle = true;
 				}
-				for (int q = f; q < l; q++) {
-					const int4 pq = p32[q];
-					const float dx = fdiff(pq.x, me.x), dy = fdiff(pq.y, me.y),
-							dz = fdiff(pq.z, me.z);
-					const float m = __int_as_float(pq.w);
-					newton32(a, m, dx, dy, dz, w);
-					if (per_particle && (dx != 0.f || dy != 0.f || dz != 0.f))
-						ewald32(a, tab, m, dx, dy, dz, w);
-					tick(a, every);
-				}
+				if (!ndone || per_particle)
+					for (int q = f; q < l; q++) {
+						const int4 pq = p32[q];
+						const float dx = fdiff(pq.x, me.x),
+								dy = fdiff(pq.y, me.y), dz = fdiff(pq.z, me.z);
+						const float m = __int_as_float(pq.w);
+						if (!ndone)
+							newton32(a, m, dx, dy, dz, w);
+						if (per_particle
+								&& (dx != 0.f || dy != 0.f || dz != 0.f))
+							ewald32(a, tab, m, dx, dy, dz, w);
+						tick(a, every);
+					}
 				node++;
 				continue;
 			}
@@ -417,10 +452,21 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 			bool open;
 			const float sx = fdiff(B.x, me.x), sy = fdiff(B.y, me.y),
 					sz = fdiff(B.z, me.z);
+			if (ndone) {
+				/* Newton taken at an ancestor: refine for the correction */
+				if (__all(ewald_ok(m, s, rx, ry, rz, sx, sy, sz, crit, w))) {
+					ewald32(a, tab, m, rx, ry, rz, w);
+					tick(a, every);
+					node += c.x;
+				} else {
+					node++;
+				}
+				continue;
+			}
 			if (w.use_bh) {
 				open = s * s > r2 * w.theta_bh;
 			} else {
-				open = m * s * s > (r2 * crit) * r2;
+				open = rel_open(m, s, r2, crit, w);
 				open = open || (fabsf(sx) < 0.6f * s && fabsf(sy) < 0.6f * s
 						&& fabsf(sz) < 0.6f * s);
 			}
@@ -428,9 +474,17 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 				newton32(a, m, rx, ry, rz, w);
 				if (w.quad)
 					quad32(a, q32[2 * node], q32[2 * node + 1].x, rx, ry, rz, w);
+				tick(a, every);
+				if (pending && !__all(ewald_ok(m, s, rx, ry, rz, sx, sy, sz,
+						crit, w))) {
+					/* the correction is not smooth enough over this node
+					 * for every lane: refine it for the correction only */
+					nw_until = node + c.x;
+					node++;
+					continue;
+				}
 				if (pending)
 					ewald32(a, tab, m, rx, ry, rz, w);
-				tick(a, every);
 				node += c.x;
 				continue;
 			}
@@ -506,7 +560,9 @@ int tdv_walk32(tdv_context *c, int use_bh)
 		c->top32 = nullptr;
 		c->tq32 = nullptr;
 		c->top32_cap = 0;
-		TDV_HIP(c, hipMalloc((void **)&c->top32, c->top_cap * sizeof(int4)));
+		/* {CoM, mass'} then {centre, size}, top_cap records each */
+		TDV_HIP(c, hipMalloc((void **)&c->top32,
+				2 * (size_t)c->top_cap * sizeof(int4)));
 		TDV_HIP(c, hipMalloc((void **)&c->tq32,
 				2 * (size_t)c->top_cap * sizeof(float4)));
 		c->top32_cap = c->top_cap;
@@ -524,7 +580,7 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	}
 	hipLaunchKernelGGL(k_prep32_top, dim3(tdv_blocks((size_t)c->ntop, 64)),
 			dim3(64), 0, c->stream, c->ntop, c->top, o[0], o[1], o[2], S,
-			mscale, c->top32);
+			mscale, c->dom_size * S, c->top32, c->top32 + c->top32_cap);
 	if (nn)
 		hipLaunchKernelGGL(k_prep32_nodes, dim3(tdv_blocks(nn, 256)),
 				dim3(256), 0, c->stream, nn, c->node_a, c->node_b, o[0], o[1],
@@ -556,10 +612,11 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	w.u_scale = (float)(2.0 * TDV_NE / 4294967296.0);
 	w.carry_every = 64;
 	w.quad = quad ? 1 : 0;
+	w.crit3 = c->par.fp32_criterion == 3;
 	hipLaunchKernelGGL(k_walk32, dim3(tdv_blocks((size_t)c->nactive, 64)),
 			dim3(64), 0, c->stream, c->nactive, c->active, c->p32, c->last_acc,
 			c->top, c->a32, c->b32, c->node_c, c->ewald32, w, c->acc[0],
 			c->acc[1], c->acc[2], c->gacc[0], c->gacc[1], c->gacc[2], c->cost,
-			c->d_count, c->top32, c->q32, c->tq32);
+			c->d_count, c->top32, c->top32 + c->top32_cap, c->q32, c->tq32);
 	return tdv_check(c, hipGetLastError(), "k_walk32");
 }

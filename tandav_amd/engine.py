@@ -36,7 +36,8 @@ class TdvParams(ctypes.Structure):
                 ("device", ctypes.c_int), ("precision", ctypes.c_int),
                 ("ewald_size", ctypes.c_double),
                 ("ewald_accuracy", ctypes.c_double),
-                ("fp32_order", ctypes.c_int)]
+                ("fp32_order", ctypes.c_int),
+                ("fp32_criterion", ctypes.c_int)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -122,7 +123,8 @@ class Engine:
     def __init__(self, periodic=False, boxsize=1.0, gravity=1.0,
                  softening=0.01, theta_bh=0.1, alpha_rel=0.02, vector_size=64,
                  compat=True, device=0, ewald_table=None, precision="fp64",
-                 ewald_size=0.125, ewald_accuracy=0.1, fp32_order=2):
+                 ewald_size=0.125, ewald_accuracy=0.1, fp32_order=2,
+                 fp32_criterion=2):
         self.lib = load_library()
         prec = {"fp64": 0, "fp32": 1}[precision]
         self.params = TdvParams(int(periodic), float(boxsize), float(gravity),
@@ -130,7 +132,7 @@ class Engine:
                                 float(alpha_rel), int(vector_size),
                                 int(compat), int(device), prec,
                                 float(ewald_size), float(ewald_accuracy),
-                                int(fp32_order))
+                                int(fp32_order), int(fp32_criterion))
         self._c = _ctx()
         rc = self.lib.tdv_create(ctypes.byref(self.params),
                                  ctypes.byref(self._c))

@@ -60,7 +60,8 @@ def _compare(h, e):
 
 @pytest.mark.parametrize("periodic", [False, True])
 def test_binding_step_matches_engine(periodic):
-    table = oracle.cached_ewald_table() if periodic else None
+    # no table handed in: the engine computes it on the GPU, as the
+    # binding's context does (Gravity_Periodic_Init)
     if periodic:
         pos, vel, m = ics.zeldovich_box(24, seed=7)
         soft, box = 1 / 960., 1.0
@@ -69,7 +70,7 @@ def test_binding_step_matches_engine(periodic):
         soft, box = 0.01, 1.0
     alpha = 0.005
     e = Engine(periodic=periodic, softening=soft, alpha_rel=alpha,
-               theta_bh=0.1, ewald_table=table)
+               theta_bh=0.1)
     e.set_particles(pos, m, vel=vel)
     e.setup_domain()
     h = glue_host.Host(periodic)
@@ -130,6 +131,7 @@ def test_binding_active_subset_and_tree_walk():
     top = _engine_step(e, True)          # all active on the prepare step
     h = glue_host.Host(False)
     h.init(pos, vel, m, softening=0.01, alpha_rel=0.005, time_bin=tb)
+    h.L.host_set_max_active_bin(10)       # a sync point: every bin active
     size, origin, _ = e.domain
     h.decomposition(size, origin, top)
     h.L.host_compute_acceleration(1)
