@@ -68,6 +68,9 @@ typedef struct {
 	                          Gravity/tree_accel.c:257), 3 = M s^3 > r^5 |a|
 	                          alpha / G (the error bound of the quadrupole
 	                          walk)                                           */
+	int fp32_carry;        /* FP32 walk: interactions summed in FP32 before
+	                          the partial sums are carried into FP64
+	                          (0 = 16)                                        */
 } tdv_params;
 
 enum { TDV_WALK_FP64 = 0, TDV_WALK_FP32 = 1 };

@@ -117,6 +117,7 @@ static void free_particles(tdv_context *c)
 	dfree(c->off);
 	dfree(c->lohi);
 	dfree(c->top.ptop);
+	dfree(c->p32);                   /* FP32 walk records, sized by cap */
 	c->cap = 0;
 }
 
@@ -254,6 +255,8 @@ int tdv_create(const tdv_params *p, tdv_context **out)
 		return TDV_ERR_ARG;
 	if (p->fp32_criterion != 0 && p->fp32_criterion != 2
 			&& p->fp32_criterion != 3)
+		return TDV_ERR_ARG;
+	if (p->fp32_carry < 0 || p->fp32_carry > 4096)
 		return TDV_ERR_ARG;
 	tdv_context *c = new (std::nothrow) tdv_context();
 	if (!c)

@@ -14,7 +14,9 @@
  * difference is rounded to FP32, so nearby particles do not lose digits to
  * cancellation.  Masses are pre-scaled so the kernel works in fixed-point
  * length units.  Lanes accumulate in FP32 and carry the partial sums into
- * FP64 accumulators every 64 interactions.
+ * FP64 accumulators every fp32_carry interactions: in a near-uniform box |a|
+ * is a small residue of large terms, and the rounding of long FP32 partial
+ * sums, not the tree, would set the error floor.
  *
  * Periodic runs add the Ewald correction in the same traversal: a node that
  * is taken as a monopole adds Newton + correction at its centre of mass
@@ -90,11 +92,8 @@ __global__ void k_prep32_nodes(size_t nn, const double4 *__restrict__ na,
 			(int)fixp(b.z, o2, S), __float_as_int((float)(b.w * S)));
 }
 
-/* top nodes: t32[t] = {CoM, mass'}, tb32[t] = {centre, size} in fixed
- * units (the float centre of domain.c:220-250) */
 __global__ void k_prep32_top(int ntop, const TopNodes T, double o0, double o1,
-		double o2, double S, double mscale, double dom_f,
-		int4 *__restrict__ t32, int4 *__restrict__ tb32)
+		double o2, double S, double mscale, int4 *__restrict__ t32)
 {
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= ntop)
@@ -103,9 +102,6 @@ __global__ void k_prep32_top(int ntop, const TopNodes T, double o0, double o1,
 	t32[t] = make_int4((int)fixp(T.com[3 * t], o0, S),
 			(int)fixp(T.com[3 * t + 1], o1, S), (int)fixp(T.com[3 * t + 2], o2, S),
 			__float_as_int((float)m));
-	tb32[t] = make_int4((int)fixp(T.pos[3 * t], o0, S),
-			(int)fixp(T.pos[3 * t + 1], o1, S), (int)fixp(T.pos[3 * t + 2], o2, S),
-			__float_as_int((float)(dom_f / (double)(1 << T.level[t]))));
 }
 
 __global__ void k_prep32_ewald(const double *__restrict__ raw,
@@ -330,6 +326,42 @@ __device__ __forceinline__ bool ewald_ok(float m, float s, float rx, float ry,
 	return m * s * s <= (d2 * (crit * w.ewald_acc)) * d2;
 }
 
+/* The Newton + Ewald pair at a node's centre of mass is the monopole of
+ * the periodic kernel, whose curvature comes from the particle's own image
+ * at d and from the other images at d + n box.  quad32 covers d; this adds
+ * the quadrupole term of the images next to it (n_k in {0, -sign d_k}, the
+ * 7 images across the nearest faces, edges and corner) wherever the node's
+ * size makes it matter by the same criterion (M s^2 > D^4 crit f, or BH
+ * s^2 > theta f D^2, f = ewald_accuracy).  Their monopoles are already
+ * inside the tabulated correction, so only the quadrupoles are added; a
+ * node straddling the half-box planes is covered the same way. */
+__device__ __forceinline__ void image_quads(Acc32 &a, const float4 q0,
+		float qyz, float m, float s, float dx, float dy, float dz, float crit,
+		const Walk32Params &w)
+{
+	const float ex = dx > 0.f ? -w.box_f : w.box_f;
+	const float ey = dy > 0.f ? -w.box_f : w.box_f;
+	const float ez = dz > 0.f ? -w.box_f : w.box_f;
+	/* nearest other image: no closer than box - max |d_k| */
+	const float dmin = w.box_f - fmaxf(fabsf(dx), fmaxf(fabsf(dy), fabsf(dz)));
+	const float lhs = w.use_bh ? s * s : m * s * s;
+	auto needed = [&](float D2) {
+		return w.use_bh ? lhs > (w.theta_bh * w.ewald_acc) * D2
+				: lhs > (D2 * (crit * w.ewald_acc)) * D2;
+	};
+	if (!needed(dmin * dmin))
+		return;
+#pragma unroll 1
+	for (int n = 1; n < 8; n++) {
+		const float ix = (n & 1) ? dx + ex : dx;
+		const float iy = (n & 2) ? dy + ey : dy;
+		const float iz = (n & 4) ? dz + ez : dz;
+		const float D2 = fmaf(ix, ix, fmaf(iy, iy, iz * iz));
+		if (needed(D2))
+			quad32(a, q0, qyz, ix, iy, iz, w);
+	}
+}
+
 __global__ void __launch_bounds__(64) k_walk32(int nact,
 		const int *__restrict__ active, const int4 *__restrict__ p32,
 		const double *__restrict__ last_acc, TopNodes T,
@@ -339,8 +371,8 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 		double *__restrict__ az, double *__restrict__ gx,
 		double *__restrict__ gy, double *__restrict__ gz,
 		double *__restrict__ cost, unsigned long long *counter,
-		const int4 *__restrict__ top32, const int4 *__restrict__ topb32,
-		const float4 *__restrict__ q32, const float4 *__restrict__ tq32)
+		const int4 *__restrict__ top32, const float4 *__restrict__ q32,
+		const float4 *__restrict__ tq32)
 {
 	const int gid = blockIdx.x * 64 + threadIdx.x;
 	const bool live = gid < nact;
@@ -363,21 +395,19 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 				: rel_open(tm, ts, r2, crit, w);
 		/* the top node holds the particle (or is close): walk it */
 		want = want || r2 < 3.f * ts * ts;
-		bool newton_top = true;       /* Newton terms still to add below */
 		if (!__any(want)) {
 			newton32(a, tm, rx, ry, rz, w);
-			if (w.quad)
-				quad32(a, tq32[2 * j], tq32[2 * j + 1].x, rx, ry, rz, w);
-			tick(a, every);
-			if (!w.periodic)
-				continue;
-			const int4 tb = topb32[j];
-			if (__all(ewald_ok(tm, ts, rx, ry, rz, fdiff(tb.x, me.x),
-					fdiff(tb.y, me.y), fdiff(tb.z, me.z), crit, w))) {
-				ewald32(a, tab, tm, rx, ry, rz, w);
-				continue;
+			if (w.quad) {
+				const float4 q0 = tq32[2 * j];
+				const float qyz = tq32[2 * j + 1].x;
+				quad32(a, q0, qyz, rx, ry, rz, w);
+				if (w.periodic)
+					image_quads(a, q0, qyz, tm, ts, rx, ry, rz, crit, w);
 			}
-			newton_top = false;       /* walk it for the correction only */
+			if (w.periodic)
+				ewald32(a, tab, tm, rx, ry, rz, w);
+			tick(a, every);
+			continue;
 		}
 		if (T.npart[j] <= w.vs) {
 			const int f = T.first[j], l = f + T.npart[j];
@@ -386,8 +416,7 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 				const float dx = fdiff(pq.x, me.x), dy = fdiff(pq.y, me.y),
 						dz = fdiff(pq.z, me.z);
 				const float m = __int_as_float(pq.w);
-				if (newton_top)
-					newton32(a, m, dx, dy, dz, w);
+				newton32(a, m, dx, dy, dz, w);
 				if (w.periodic && (dx != 0.f || dy != 0.f || dz != 0.f))
 					ewald32(a, tab, m, dx, dy, dz, w);
 				tick(a, every);
@@ -397,17 +426,11 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 		const int start = T.target[j];
 		int node = start;
 		int ew_until = w.periodic ? -1 : 0x7FFFFFFF;  /* correction done below */
-		int nw_until = newton_top ? -1 : 0x7FFFFFFF;  /* Newton done below */
 		for (;;) {
 			const int4 c = nc[node];
 			if (c.x == 0 && node != start)
 				break;
 			const bool pending = node >= ew_until;
-			const bool ndone = node < nw_until;
-			if (ndone && !pending) {      /* nothing left in this subtree */
-				node += c.x > 0 ? c.x : 1;
-				continue;
-			}
 			if (c.x < 0) {
 				const int f = -c.x - 1, l = f + c.y;
 				bool per_particle = false;
@@ -427,19 +450,16 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 					else
 						per_particle = true;
 				}
-				if (!ndone || per_particle)
-					for (int q = f; q < l; q++) {
-						const int4 pq = p32[q];
-						const float dx = fdiff(pq.x, me.x),
-								dy = fdiff(pq.y, me.y), dz = fdiff(pq.z, me.z);
-						const float m = __int_as_float(pq.w);
-						if (!ndone)
-							newton32(a, m, dx, dy, dz, w);
-						if (per_particle
-								&& (dx != 0.f || dy != 0.f || dz != 0.f))
-							ewald32(a, tab, m, dx, dy, dz, w);
-						tick(a, every);
-					}
+				for (int q = f; q < l; q++) {
+					const int4 pq = p32[q];
+					const float dx = fdiff(pq.x, me.x), dy = fdiff(pq.y, me.y),
+							dz = fdiff(pq.z, me.z);
+					const float m = __int_as_float(pq.w);
+					newton32(a, m, dx, dy, dz, w);
+					if (per_particle && (dx != 0.f || dy != 0.f || dz != 0.f))
+						ewald32(a, tab, m, dx, dy, dz, w);
+					tick(a, every);
+				}
 				node++;
 				continue;
 			}
@@ -452,17 +472,6 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 			bool open;
 			const float sx = fdiff(B.x, me.x), sy = fdiff(B.y, me.y),
 					sz = fdiff(B.z, me.z);
-			if (ndone) {
-				/* Newton taken at an ancestor: refine for the correction */
-				if (__all(ewald_ok(m, s, rx, ry, rz, sx, sy, sz, crit, w))) {
-					ewald32(a, tab, m, rx, ry, rz, w);
-					tick(a, every);
-					node += c.x;
-				} else {
-					node++;
-				}
-				continue;
-			}
 			if (w.use_bh) {
 				open = s * s > r2 * w.theta_bh;
 			} else {
@@ -472,19 +481,16 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 			}
 			if (!__any(open)) {
 				newton32(a, m, rx, ry, rz, w);
-				if (w.quad)
-					quad32(a, q32[2 * node], q32[2 * node + 1].x, rx, ry, rz, w);
-				tick(a, every);
-				if (pending && !__all(ewald_ok(m, s, rx, ry, rz, sx, sy, sz,
-						crit, w))) {
-					/* the correction is not smooth enough over this node
-					 * for every lane: refine it for the correction only */
-					nw_until = node + c.x;
-					node++;
-					continue;
+				if (w.quad) {
+					const float4 q0 = q32[2 * node];
+					const float qyz = q32[2 * node + 1].x;
+					quad32(a, q0, qyz, rx, ry, rz, w);
+					if (pending)
+						image_quads(a, q0, qyz, m, s, rx, ry, rz, crit, w);
 				}
 				if (pending)
 					ewald32(a, tab, m, rx, ry, rz, w);
+				tick(a, every);
 				node += c.x;
 				continue;
 			}
@@ -560,9 +566,7 @@ int tdv_walk32(tdv_context *c, int use_bh)
 		c->top32 = nullptr;
 		c->tq32 = nullptr;
 		c->top32_cap = 0;
-		/* {CoM, mass'} then {centre, size}, top_cap records each */
-		TDV_HIP(c, hipMalloc((void **)&c->top32,
-				2 * (size_t)c->top_cap * sizeof(int4)));
+		TDV_HIP(c, hipMalloc((void **)&c->top32, c->top_cap * sizeof(int4)));
 		TDV_HIP(c, hipMalloc((void **)&c->tq32,
 				2 * (size_t)c->top_cap * sizeof(float4)));
 		c->top32_cap = c->top_cap;
@@ -580,7 +584,7 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	}
 	hipLaunchKernelGGL(k_prep32_top, dim3(tdv_blocks((size_t)c->ntop, 64)),
 			dim3(64), 0, c->stream, c->ntop, c->top, o[0], o[1], o[2], S,
-			mscale, c->dom_size * S, c->top32, c->top32 + c->top32_cap);
+			mscale, c->top32);
 	if (nn)
 		hipLaunchKernelGGL(k_prep32_nodes, dim3(tdv_blocks(nn, 256)),
 				dim3(256), 0, c->stream, nn, c->node_a, c->node_b, o[0], o[1],
@@ -610,13 +614,13 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	w.ewald_size = (float)(c->par.ewald_size * c->par.boxsize * S);
 	w.ewald_acc = (float)c->par.ewald_accuracy;
 	w.u_scale = (float)(2.0 * TDV_NE / 4294967296.0);
-	w.carry_every = 64;
+	w.carry_every = c->par.fp32_carry > 0 ? c->par.fp32_carry : 16;
 	w.quad = quad ? 1 : 0;
 	w.crit3 = c->par.fp32_criterion == 3;
 	hipLaunchKernelGGL(k_walk32, dim3(tdv_blocks((size_t)c->nactive, 64)),
 			dim3(64), 0, c->stream, c->nactive, c->active, c->p32, c->last_acc,
 			c->top, c->a32, c->b32, c->node_c, c->ewald32, w, c->acc[0],
 			c->acc[1], c->acc[2], c->gacc[0], c->gacc[1], c->gacc[2], c->cost,
-			c->d_count, c->top32, c->top32 + c->top32_cap, c->q32, c->tq32);
+			c->d_count, c->top32, c->q32, c->tq32);
 	return tdv_check(c, hipGetLastError(), "k_walk32");
 }
