@@ -145,6 +145,14 @@ int tdv_compute_acceleration(tdv_context *ctx, int prepare_step);
  * order; acc_out = 3*nt doubles. */
 int tdv_forcetest(tdv_context *ctx, int nt, const int *idx, double *acc_out);
 
+/* tdv_forcetest with the Ewald correction of every pair summed exactly
+ * (the Hernquist, Bouchet & Suto sums of Gravity/periodic.c:352-403
+ * evaluated at the pair's separation) instead of read from the 65^3 table:
+ * the periodic force without the table's interpolation error, for stating
+ * the walks' error.  The same as tdv_forcetest on non-periodic runs. */
+int tdv_forcetest_exact(tdv_context *ctx, int nt, const int *idx,
+		double *acc_out);
+
 /* Element-wise batched forms of the pure functions (host arrays). */
 /* Peano_Key / Reversed_Peano_Key (peano.c:122-276); keys_out = 2n words. */
 int tdv_peano_keys(tdv_context *ctx, size_t n, const double *x,
@@ -165,6 +173,14 @@ int tdv_ewald_correction(tdv_context *ctx, size_t n, const double *dr,
  * unnormalised 4 x 65^3 table (Fx, Fy, Fz, Fp) in box units. */
 int tdv_get_ewald_table(tdv_context *ctx, double *table4);
 int tdv_set_ewald_table(tdv_context *ctx, const double *table4);
+
+/* Second derivatives S_abc = d_b d_c E_a of the Ewald correction E
+ * (Ewald_Correction's function, Gravity/periodic.c:35-113, continued past
+ * the half-box faces) at the 65^3 table nodes, per unit box: 10 x 65^3
+ * doubles, components xxx yyy zzz xxy xxz xyy yyz xzz yzz xyz.  The FP32
+ * walk uses them to take a node's correction at its centre of mass to
+ * second order; the reference has no counterpart. */
+int tdv_get_ewald_derivs(tdv_context *ctx, double *s10);
 
 /* Domain (domain.h:58-63) after the last decomposition. */
 int tdv_get_domain(tdv_context *ctx, double *size, double origin[3],

@@ -304,7 +304,8 @@ void tdv_destroy(tdv_context *c)
 	dfree(c->node_c);
 	dfree(c->node_up);
 	dfree(c->ewald);
-	dfree(c->ewald32);
+	dfree(c->ewald16);
+	dfree(c->ewald_s);
 	dfree(c->p32);
 	dfree(c->a32);
 	dfree(c->b32);
@@ -664,7 +665,22 @@ int tdv_compute_acceleration(tdv_context *c, int prepare_step)
 	return tdv_tree_acceleration(c, 0);
 }
 
+static int forcetest(tdv_context *c, int nt, const int *idx, double *acc_out,
+		bool exact);
+
 int tdv_forcetest(tdv_context *c, int nt, const int *idx, double *acc_out)
+{
+	return forcetest(c, nt, idx, acc_out, false);
+}
+
+int tdv_forcetest_exact(tdv_context *c, int nt, const int *idx,
+		double *acc_out)
+{
+	return forcetest(c, nt, idx, acc_out, true);
+}
+
+static int forcetest(tdv_context *c, int nt, const int *idx, double *acc_out,
+		bool exact)
 {
 	if (!c || nt < 0 || (nt && (!idx || !acc_out)) || !c->n)
 		return tdv_fail(c, TDV_ERR_ARG, "tdv_forcetest: bad arguments");
@@ -682,7 +698,8 @@ int tdv_forcetest(tdv_context *c, int nt, const int *idx, double *acc_out)
 		r = tdv_check(c, hipMemcpyAsync(di, idx, nt * sizeof(int),
 				hipMemcpyHostToDevice, c->stream), "copy");
 	if (!r)
-		r = tdv_direct_sum(c, nt, di, dout);
+		r = exact ? tdv_direct_sum_exact(c, nt, di, dout)
+				: tdv_direct_sum(c, nt, di, dout);
 	if (!r)
 		r = tdv_check(c, hipMemcpyAsync(acc_out, dout, 3 * nt * sizeof(double),
 				hipMemcpyDeviceToHost, c->stream), "copy");
@@ -809,7 +826,21 @@ int tdv_set_ewald_table(tdv_context *c, const double *t4)
 			hipMemcpyHostToDevice, c->stream));
 	TDV_HIP(c, hipStreamSynchronize(c->stream));
 	c->have_ewald = 1;
+	c->ewald16_ok = 0;               /* the FP32 records follow the table */
 	return 0;
+}
+
+int tdv_get_ewald_derivs(tdv_context *c, double *s10)
+{
+	if (!c || !s10 || !c->par.periodic)
+		return tdv_fail(c, TDV_ERR_ARG, "no Ewald derivatives on this run");
+	if (!c->ewald_s) {
+		TDV_TRY(dalloc(c, &c->ewald_s, 10 * (size_t)TDV_NE3));
+		TDV_TRY(tdv_ewald_derivs_device(c, c->ewald_s));
+	}
+	TDV_HIP(c, hipMemcpyAsync(s10, c->ewald_s, 10 * (size_t)TDV_NE3 * 8,
+			hipMemcpyDeviceToHost, c->stream));
+	return tdv_check(c, hipStreamSynchronize(c->stream), "ewald derivs");
 }
 
 int tdv_get_domain(tdv_context *c, double *size, double origin[3],

@@ -76,3 +76,203 @@ int tdv_ewald_table_device(tdv_context *c, double *out4)
 			out4 + 3 * TDV_NE3);
 	return tdv_check(c, hipGetLastError(), "k_ewald_table");
 }
+
+/*
+ * Second derivatives of the Ewald correction, for taking a node's
+ * correction at its centre of mass to second order (the FP32 group walk).
+ *
+ * F(x) is the table's function for any x, not only on the octant grid: the
+ * same real-space (|n| <= 4) and Fourier (|h| <= 4) sums as k_ewald_table,
+ * alpha = 2, with the n = 0 real-space term and the Newton term x / r^3
+ * combined as x (erf(a r) - 2 a r / sqrt(pi) exp(-a^2 r^2)) / r^3, a series
+ * below a r = 0.5, so F is smooth through x = 0.  The walk's correction is
+ * E(d) = -F(d) (the table's sign rule), continued smoothly past the half-box
+ * faces (the next singularity is the image at distance box).  Its second
+ * derivatives S_abc = d_b d_c E_a are symmetric in (a, b, c) and traceless
+ * in (b, c) away from the images; the ten independent ones come from central
+ * differences of F with step H (box units): nine pure second differences
+ * (three evaluations per axis) and d_y d_z E_x from four diagonal ones.
+ * Layout: 10 x 65^3 doubles, component-major, in the order
+ * xxx yyy zzz xxy xxz xyy yyz xzz yzz xyz, per unit box.
+ */
+#define EW_H (1.0 / 512.0)
+
+__device__ void ewald_F_any(const double x[3], double f[3])
+{
+	const double alpha = 2.0;
+	const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+	const double r = sqrt(r2);
+	const double z = alpha * r;
+	/* x g(z) / r^3 with g(z) = erf(z) - 2 z / sqrt(pi) exp(-z^2) */
+	double gr3;
+	if (z < 0.5) {
+		/* g(z) = 2/sqrt(pi) sum_{n>=1} (-1)^(n+1) 2n z^(2n+1) / (n! (2n+1)) */
+		const double z2 = z * z;
+		double term = 1.0, sum = 0.0, zp = 1.0;  /* zp = z^(2n-2) */
+		for (int n = 1; n <= 14; n++) {
+			term = (n == 1) ? 1.0 : term / n;  /* 1 / n! */
+			const double c = ((n & 1) ? 1.0 : -1.0) * 2.0 * n / (2.0 * n + 1.0);
+			sum += c * term * zp;
+			zp *= z2;
+		}
+		/* g / r^3 = 2/sqrt(pi) alpha^3 sum_n c_n z^(2n-2) / n! */
+		gr3 = 2.0 / sqrt(EW_PI) * alpha * alpha * alpha * sum;
+	} else {
+		gr3 = (erf(z) - 2.0 * z / sqrt(EW_PI) * exp(-z * z)) / (r2 * r);
+	}
+	for (int d = 0; d < 3; d++)
+		f[d] = x[d] * gr3;
+	for (int a = -4; a < 5; a++)
+		for (int b = -4; b < 5; b++)
+			for (int c = -4; c < 5; c++) {
+				if (a == 0 && b == 0 && c == 0)
+					continue;
+				const double dx[3] = { x[0] - a, x[1] - b, x[2] - c };
+				const double rr = sqrt(dx[0] * dx[0] + dx[1] * dx[1]
+						+ dx[2] * dx[2]);
+				const double val = erfc(alpha * rr) + 2 * alpha * rr
+						/ sqrt(EW_PI) * exp(-((alpha * rr) * (alpha * rr)));
+				for (int d = 0; d < 3; d++)
+					f[d] -= dx[d] / (rr * rr * rr) * val;
+			}
+	for (int a = -4; a < 5; a++)
+		for (int b = -4; b < 5; b++)
+			for (int c = -4; c < 5; c++) {
+				const int h2 = a * a + b * b + c * c;
+				if (h2 == 0)
+					continue;
+				const double hx = x[0] * a + x[1] * b + x[2] * c;
+				const double e = exp(-EW_PI * EW_PI * h2 / (alpha * alpha));
+				const double val = 2.0 / h2 * e * sin(2 * EW_PI * hx);
+				f[0] -= a * val;
+				f[1] -= b * val;
+				f[2] -= c * val;
+			}
+}
+
+__global__ void __launch_bounds__(64) k_ewald_derivs(double *__restrict__ s)
+{
+	const int n = blockIdx.x * blockDim.x + threadIdx.x;
+	if (n >= TDV_NE3)
+		return;
+	const int i = n / (TDV_NE1 * TDV_NE1), j = (n / TDV_NE1) % TDV_NE1,
+			k = n % TDV_NE1;
+	const double x0[3] = { (0.5 * i) / TDV_NE, (0.5 * j) / TDV_NE,
+		(0.5 * k) / TDV_NE };
+	const double H = EW_H;
+	double c0[3], p[3], m[3];
+	ewald_F_any(x0, c0);
+	double d2[3][3];                         /* d2[axis][component] of F */
+	for (int ax = 0; ax < 3; ax++) {
+		double xp[3] = { x0[0], x0[1], x0[2] }, xm[3] = { x0[0], x0[1], x0[2] };
+		xp[ax] += H;
+		xm[ax] -= H;
+		ewald_F_any(xp, p);
+		ewald_F_any(xm, m);
+		for (int q = 0; q < 3; q++)
+			d2[ax][q] = (p[q] - 2.0 * c0[q] + m[q]) / (H * H);
+	}
+	double pp[3], pm[3], mp[3], mm[3];
+	{
+		double x1[3] = { x0[0], x0[1] + H, x0[2] + H };
+		ewald_F_any(x1, pp);
+		double x2[3] = { x0[0], x0[1] + H, x0[2] - H };
+		ewald_F_any(x2, pm);
+		double x3[3] = { x0[0], x0[1] - H, x0[2] + H };
+		ewald_F_any(x3, mp);
+		double x4[3] = { x0[0], x0[1] - H, x0[2] - H };
+		ewald_F_any(x4, mm);
+	}
+	const double dyz_x = (pp[0] - pm[0] - mp[0] + mm[0]) / (4.0 * H * H);
+	/* E = -F */
+	const double v[10] = { -d2[0][0], -d2[1][1], -d2[2][2], -d2[0][1],
+		-d2[0][2], -d2[1][0], -d2[1][2], -d2[2][0], -d2[2][1], -dyz_x };
+	for (int q = 0; q < 10; q++)
+		s[(size_t)q * TDV_NE3 + n] = v[q];
+}
+
+int tdv_ewald_derivs_device(tdv_context *c, double *out10)
+{
+	hipLaunchKernelGGL(k_ewald_derivs, dim3(tdv_blocks(TDV_NE3, 64)), dim3(64),
+			0, c->stream, out10);
+	return tdv_check(c, hipGetLastError(), "k_ewald_derivs");
+}
+
+/*
+ * FP64 direct summation with the exact Ewald correction of every pair: the
+ * same pairwise Newton kernel as k_direct (nearest image, K1 softening),
+ * plus -F(d / Box) / Box^2 from the Ewald sums themselves (ewald_F_any)
+ * instead of the 65^3 table's trilinear interpolation.  An error oracle for
+ * the walks that is free of the table's interpolation error; one workgroup
+ * per target, lanes over the sources.
+ */
+__global__ void __launch_bounds__(256) k_direct_exact(int nt,
+		const int *__restrict__ idx, const double4 *__restrict__ pm, int n,
+		double box, double G, double eps2, double eps3,
+		double *__restrict__ out)
+{
+	__shared__ double red[3][256];
+	const int t = blockIdx.x;
+	if (t >= nt)
+		return;
+	const int i = idx[t];
+	const double4 p = pm[i];
+	const double half = box / 2.0, ib = 1.0 / box;
+	double ax = 0, ay = 0, az = 0;
+	for (int j = threadIdx.x; j < n; j += blockDim.x) {
+		if (j == i)
+			continue;
+		const double4 q = pm[j];
+		double d[3] = { q.x - p.x, q.y - p.y, q.z - p.z };
+		for (int k = 0; k < 3; k++) {
+			if (d[k] > half)
+				d[k] -= box;
+			else if (d[k] < -half)
+				d[k] += box;
+		}
+		const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+		if (r2 == 0)
+			continue;
+		double fac = G * q.w;
+		if (r2 < eps2) {
+			const double u2 = r2 / eps2;
+			fac *= (175 - u2 * (294 - u2 * 135)) / (16 * eps3);
+		} else {
+			const double ri = 1 / sqrt(r2);
+			fac *= ri * ri * ri;
+		}
+		const double du[3] = { d[0] * ib, d[1] * ib, d[2] * ib };
+		double f[3];
+		ewald_F_any(du, f);
+		const double ge = G * q.w * ib * ib;
+		ax += fac * d[0] - ge * f[0];
+		ay += fac * d[1] - ge * f[1];
+		az += fac * d[2] - ge * f[2];
+	}
+	red[0][threadIdx.x] = ax;
+	red[1][threadIdx.x] = ay;
+	red[2][threadIdx.x] = az;
+	__syncthreads();
+	for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+		if ((int)threadIdx.x < s)
+			for (int d = 0; d < 3; d++)
+				red[d][threadIdx.x] += red[d][threadIdx.x + s];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		for (int d = 0; d < 3; d++)
+			out[3 * t + d] = red[d][0];
+}
+
+int tdv_direct_sum_exact(tdv_context *c, int nt, const int *idx_dev,
+		double *out_dev)
+{
+	if (nt <= 0)
+		return 0;
+	if (!c->par.periodic)
+		return tdv_direct_sum(c, nt, idx_dev, out_dev);
+	hipLaunchKernelGGL(k_direct_exact, dim3(nt), dim3(256), 0, c->stream, nt,
+			idx_dev, c->pm, (int)c->n, c->par.boxsize, c->par.gravity,
+			c->Epsilon2, c->Epsilon3, out_dev);
+	return tdv_check(c, hipGetLastError(), "k_direct_exact");
+}

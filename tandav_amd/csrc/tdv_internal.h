@@ -99,7 +99,11 @@ struct tdv_context {
 	size_t n32_cap;
 	float4 *q32;                     /* 2 per node: traceless quadrupole */
 	float4 *tq32;                    /* 2 per top node */
-	float *ewald32;
+	double *ewald_s;                 /* 10 * TDV_NE3: second derivatives
+	                                    of the correction, per unit box */
+	float4 *ewald16;                 /* FP32 walk: 4 float4 per grid node,
+	                                    {E, S} in kernel units */
+	int ewald16_ok;                  /* ewald16 matches ewald */
 
 	unsigned *d_err;
 	unsigned long long *d_count;     /* [2] interactions: tree, Ewald */
@@ -140,8 +144,11 @@ int tdv_walk64(tdv_context *c, int use_bh);
 int tdv_walk64_ewald(tdv_context *c, int use_bh);
 int tdv_walk32(tdv_context *c, int use_bh);
 int tdv_direct_sum(tdv_context *c, int nt, const int *idx_dev, double *out_dev);
+int tdv_direct_sum_exact(tdv_context *c, int nt, const int *idx_dev,
+		double *out_dev);
 void tdv_launch_safe_last_accel(tdv_context *c);
 int tdv_ewald_table_device(tdv_context *c, double *out4_unnormalised);
+int tdv_ewald_derivs_device(tdv_context *c, double *out10);
 void tdv_launch_ewald_correction(tdv_context *c, size_t n, const double *dr,
 		double *f);
 

@@ -41,6 +41,9 @@ struct Walk32Params {
 	float box_f;                     /* box in fixed units */
 	float dom_f;                     /* domain size in fixed units */
 	float u_scale;                   /* fixed units -> table grid units */
+	float inv_box_f;                 /* 1 / box_f */
+	float inv_s2;                    /* 1 / S^2: quadrupoles -> table units */
+	float tri_corr;                  /* 3 (Box h)^2: trilinear error from S */
 	int carry_every;
 	int quad;                        /* accepted nodes add their quadrupole */
 	int crit3;                       /* open if M s^3 > r^5 |a| alpha / G */
@@ -104,17 +107,34 @@ __global__ void k_prep32_top(int ntop, const TopNodes T, double o0, double o1,
 			__float_as_int((float)m));
 }
 
-__global__ void k_prep32_ewald(const double *__restrict__ raw,
-		float *__restrict__ t32, double scale)
+/* The walk's Ewald records, one per table node (4 float4 = 64 bytes, one
+ * cache line per trilinear corner): {Ex, Ey, Ez, Sxxx}, {Syyy, Szzz, Sxxy,
+ * Sxxz}, {Sxyy, Syyz, Sxzz, Syzz}, {Sxyz, 0, 0, 0}.  E = -F (the table's
+ * sign rule: Ewald_Correction returns -F(|d|) on the positive octant) in
+ * units where m' E is an acceleration (m' = G m S^2); S = d d E / 6 in the
+ * units that T'' = T' / S^2 contracts to an acceleration (the factor 1/6
+ * turns the traceless T into the second moment: Q:S/2 = T:S/6). */
+__global__ void k_prep32_ewald16(const double *__restrict__ raw,
+		const double *__restrict__ s10, double e_scale, double s_scale,
+		float4 *__restrict__ t16)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < 3 * TDV_NE3)
-		t32[i] = (float)(raw[i] * scale);
+	if (i >= TDV_NE3)
+		return;
+	float v[16];
+	for (int d = 0; d < 3; d++)
+		v[d] = (float)(-raw[(size_t)d * TDV_NE3 + i] * e_scale);
+	for (int q = 0; q < 10; q++)
+		v[3 + q] = (float)(s10[(size_t)q * TDV_NE3 + i] * s_scale);
+	v[13] = v[14] = v[15] = 0.f;
+	for (int r = 0; r < 4; r++)
+		t16[4 * i + r] = make_float4(v[4 * r], v[4 * r + 1], v[4 * r + 2],
+				v[4 * r + 3]);
 }
 
 /* Traceless second moment T = sum m (3 y y^T - |y|^2 I), y = x - CoM, of
- * the particles [first, first + npart) of every internal node (and of every
- * top node), in FP64 about the FP64 centre of mass, one wavefront per node,
+ * the particles [first, first + npart) of every node and bundle (and of
+ * every top node), in FP64 about the FP64 centre of mass, one wavefront per node,
  * stored scaled (G S^2 masses, fixed lengths) as {Txx, Tyy, Txy, Txz},
  * {Tyz, 0, 0, 0}; Tzz = -Txx - Tyy. */
 __device__ __forceinline__ void quad_moments(const double4 *__restrict__ pm,
@@ -159,7 +179,7 @@ __global__ void __launch_bounds__(256) k_prep32_quad(int nn,
 	if (j >= nn)
 		return;
 	const int4 c = nc[j];
-	if (c.x <= 0) {                  /* bundles and terminators: never used */
+	if (c.x == 0) {                  /* terminators: never used */
 		if (lane < 2)
 			q[2 * j + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
 		return;
@@ -248,46 +268,116 @@ __device__ __forceinline__ void quad32(Acc32 &a, const float4 qa, float qyz,
 	a.z = fmaf(ri4, fmaf(c, uz, -tuz), a.z);
 }
 
-/* Ewald_Correction (Gravity/periodic.c:35-113) on the FP32 table, already
- * scaled to fixed units and divided by the mass scale */
-__device__ __forceinline__ void ewald32(Acc32 &a, const float *__restrict__ t,
+/* Trilinear corners of the Ewald table for a separation d (fixed units):
+ * the positive-octant cell, fractions, and the reflection signs (the
+ * correction is odd in each axis, sign(0) = +1; Gravity/periodic.c:35-113). */
+struct EwaldCell {
+	int b;                           /* corner (i, j, k) */
+	float fu, fv, fq;
+	float sx, sy, sz;
+};
+
+__device__ __forceinline__ EwaldCell ewald_cell(float dx, float dy, float dz,
+		const Walk32Params &w)
+{
+	EwaldCell c;
+	c.sx = dx < 0.f ? -1.f : 1.f;
+	c.sy = dy < 0.f ? -1.f : 1.f;
+	c.sz = dz < 0.f ? -1.f : 1.f;
+	float u = fabsf(dx) * w.u_scale, v = fabsf(dy) * w.u_scale,
+			q = fabsf(dz) * w.u_scale;
+	const int i = min((int)u, TDV_NE - 1), j = min((int)v, TDV_NE - 1),
+			k = min((int)q, TDV_NE - 1);
+	c.fu = u - (float)i;
+	c.fv = v - (float)j;
+	c.fq = q - (float)k;
+	c.b = (i * TDV_NE1 + j) * TDV_NE1 + k;
+	return c;
+}
+
+/* Ewald_Correction (Gravity/periodic.c:35-113) of a point mass m' */
+__device__ __forceinline__ void ewald16(Acc32 &a, const float4 *__restrict__ t,
 		float m, float dx, float dy, float dz, const Walk32Params &w)
 {
-	float sx = -1.f, sy = -1.f, sz = -1.f;
-	if (dx < 0.f) {
-		dx = -dx;
-		sx = 1.f;
-	}
-	if (dy < 0.f) {
-		dy = -dy;
-		sy = 1.f;
-	}
-	if (dz < 0.f) {
-		dz = -dz;
-		sz = 1.f;
-	}
-	float u = dx * w.u_scale, v = dy * w.u_scale, q = dz * w.u_scale;
-	int i = min((int)u, TDV_NE - 1), j = min((int)v, TDV_NE - 1),
-			k = min((int)q, TDV_NE - 1);
-	u -= (float)i;
-	v -= (float)j;
-	q -= (float)k;
-	const float w0 = (1.f - u) * (1.f - v), w1 = (1.f - u) * v,
-			w2 = u * (1.f - v), w3 = u * v;
-	const int b = (i * TDV_NE1 + j) * TDV_NE1 + k;
-	const int di = TDV_NE1 * TDV_NE1, dj = TDV_NE1;
-	float f[3];
+	const EwaldCell c = ewald_cell(dx, dy, dz, w);
+	float ex = 0.f, ey = 0.f, ez = 0.f;
 #pragma unroll
-	for (int d = 0; d < 3; d++) {
-		const float *F = t + d * TDV_NE3 + b;
-		const float lo = w0 * F[0] + w1 * F[dj] + w2 * F[di] + w3 * F[di + dj];
-		const float hi = w0 * F[1] + w1 * F[dj + 1] + w2 * F[di + 1]
-				+ w3 * F[di + dj + 1];
-		f[d] = fmaf(q, hi - lo, lo);
+	for (int n = 0; n < 8; n++) {
+		const int di = n >> 2, dj = (n >> 1) & 1, dk = n & 1;
+		const float wt = (di ? c.fu : 1.f - c.fu) * (dj ? c.fv : 1.f - c.fv)
+				* (dk ? c.fq : 1.f - c.fq);
+		const float4 r0 = t[4 * (c.b + (di * TDV_NE1 + dj) * TDV_NE1 + dk)];
+		ex = fmaf(wt, r0.x, ex);
+		ey = fmaf(wt, r0.y, ey);
+		ez = fmaf(wt, r0.z, ez);
 	}
-	a.x = fmaf(m * sx, f[0], a.x);
-	a.y = fmaf(m * sy, f[1], a.y);
-	a.z = fmaf(m * sz, f[2], a.z);
+	a.x = fmaf(m * c.sx, ex, a.x);
+	a.y = fmaf(m * c.sy, ey, a.y);
+	a.z = fmaf(m * c.sz, ez, a.z);
+	a.ewald++;
+}
+
+/* The correction of a node of mass m' and traceless second moment T'
+ * {Txx, Tyy, Txy, Txz}, Tyz about its centre of mass at d, to second order:
+ * m' E(d) + T'':S(d), T'' = T'/S^2 (S = d d E / 6 in table units).  S is
+ * symmetric in its three indices and traceless in the last two, so the
+ * trace part of the second moment adds nothing. */
+__device__ __forceinline__ void ewald16q(Acc32 &a, const float4 *__restrict__ t,
+		float m, const float4 qa, float qyz, float dx, float dy, float dz,
+		const Walk32Params &w)
+{
+	const EwaldCell c = ewald_cell(dx, dy, dz, w);
+	float e[3] = { 0.f, 0.f, 0.f }, S[10];
+#pragma unroll
+	for (int q = 0; q < 10; q++)
+		S[q] = 0.f;
+#pragma unroll
+	for (int n = 0; n < 8; n++) {
+		const int di = n >> 2, dj = (n >> 1) & 1, dk = n & 1;
+		const float wt = (di ? c.fu : 1.f - c.fu) * (dj ? c.fv : 1.f - c.fv)
+				* (dk ? c.fq : 1.f - c.fq);
+		const float4 *r = t + 4 * (c.b + (di * TDV_NE1 + dj) * TDV_NE1 + dk);
+		const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+		e[0] = fmaf(wt, r0.x, e[0]);
+		e[1] = fmaf(wt, r0.y, e[1]);
+		e[2] = fmaf(wt, r0.z, e[2]);
+		S[0] = fmaf(wt, r0.w, S[0]);
+		S[1] = fmaf(wt, r1.x, S[1]);
+		S[2] = fmaf(wt, r1.y, S[2]);
+		S[3] = fmaf(wt, r1.z, S[3]);
+		S[4] = fmaf(wt, r1.w, S[4]);
+		S[5] = fmaf(wt, r2.x, S[5]);
+		S[6] = fmaf(wt, r2.y, S[6]);
+		S[7] = fmaf(wt, r2.z, S[7]);
+		S[8] = fmaf(wt, r2.w, S[8]);
+		S[9] = fmaf(wt, r3.x, S[9]);
+	}
+	/* trilinear interpolation overstates E by u(1-u) h^2 d_k^2 E / 2 per
+	 * axis; S holds those second derivatives (times 1/6 in table units) */
+	{
+		const float gu = c.fu * (1.f - c.fu), gv = c.fv * (1.f - c.fv),
+				gq = c.fq * (1.f - c.fq);
+		e[0] -= w.tri_corr * fmaf(gu, S[0], fmaf(gv, S[5], gq * S[7]));
+		e[1] -= w.tri_corr * fmaf(gu, S[3], fmaf(gv, S[1], gq * S[8]));
+		e[2] -= w.tri_corr * fmaf(gu, S[4], fmaf(gv, S[6], gq * S[2]));
+	}
+	/* reflect: S_abc is odd in an axis that appears an odd number of times */
+	const float sxxx = c.sx * S[0], syyy = c.sy * S[1], szzz = c.sz * S[2],
+			sxxy = c.sy * S[3], sxxz = c.sz * S[4], sxyy = c.sx * S[5],
+			syyz = c.sz * S[6], sxzz = c.sx * S[7], syzz = c.sy * S[8],
+			sxyz = c.sx * c.sy * c.sz * S[9];
+	const float txx = qa.x * w.inv_s2, tyy = qa.y * w.inv_s2,
+			tzz = -(txx + tyy), txy = 2.f * qa.z * w.inv_s2,
+			txz = 2.f * qa.w * w.inv_s2, tyz = 2.f * qyz * w.inv_s2;
+	const float qx = txx * sxxx + tyy * sxyy + tzz * sxzz + txy * sxxy
+			+ txz * sxxz + tyz * sxyz;
+	const float qy = txx * sxxy + tyy * syyy + tzz * syzz + txy * sxyy
+			+ txz * sxyz + tyz * syyz;
+	const float qz = txx * sxxz + tyy * syyz + tzz * szzz + txy * sxyz
+			+ txz * sxzz + tyz * syzz;
+	a.x += fmaf(m * c.sx, e[0], qx);
+	a.y += fmaf(m * c.sy, e[1], qy);
+	a.z += fmaf(m * c.sz, e[2], qz);
 	a.ewald++;
 }
 
@@ -305,68 +395,41 @@ __device__ __forceinline__ bool straddles(float sx, float sy, float sz,
 	return fabsf(sx) > lim || fabsf(sy) > lim || fabsf(sz) > lim;
 }
 
-/* Can a node's Ewald correction be taken at its centre of mass?  The
- * correction is smooth inside the nearest-image cell, but its curvature
- * grows like that of the Newton force of the nearest other image, at
- * distance d_img >= box - max|d_k|.  The monopole error is therefore bounded
- * like the tree's own quadrupole error with d_img in place of r: the same
- * relative criterion (M s^2 <= d_img^4 |a| alpha / G) or BH criterion
- * (s^2 <= theta d_img^2) decides, plus the straddle test and a size cap. */
+/* Can a node's Ewald correction be taken at its centre of mass (to second
+ * order, ewald16q)?  What is left is the third-order term, M s^3 |E'''|,
+ * with |E'''| <= K / box^5, K = 60 + 24 (box / d_img)^5: the correction's
+ * own curvature grows about linearly from the origin (|E''| ~ 58 |d| /
+ * box^4 measured on the table) and near the half-box faces like the Newton
+ * force of the nearest other image, at d_img >= box - max |d_k|.  It must
+ * stay below ewald_accuracy times the relative criterion's bound; the BH
+ * walk of the prepare step only caps the size.  Nodes whose subtree then
+ * takes Newton terms per lane must not straddle the half-box planes (their
+ * particles must share the centre's periodic image); bundles take their
+ * Newton terms in the centre's frame and may straddle. */
 __device__ __forceinline__ bool ewald_ok(float m, float s, float rx, float ry,
-		float rz, float sx, float sy, float sz, float crit,
+		float rz, float sx, float sy, float sz, float crit, bool may_straddle,
 		const Walk32Params &w)
 {
-	if (s > w.ewald_size || straddles(sx, sy, sz, s, w))
+	if (s > w.ewald_size)
 		return false;
-	const float dimg = w.box_f - fmaxf(fabsf(rx), fmaxf(fabsf(ry),
-			fabsf(rz)));
-	const float d2 = dimg * dimg;
+	if (!may_straddle && straddles(sx, sy, sz, s, w))
+		return false;
 	if (w.use_bh)
-		return s * s <= (w.theta_bh * w.ewald_acc) * d2;
-	return m * s * s <= (d2 * (crit * w.ewald_acc)) * d2;
-}
-
-/* The Newton + Ewald pair at a node's centre of mass is the monopole of
- * the periodic kernel, whose curvature comes from the particle's own image
- * at d and from the other images at d + n box.  quad32 covers d; this adds
- * the quadrupole term of the images next to it (n_k in {0, -sign d_k}, the
- * 7 images across the nearest faces, edges and corner) wherever the node's
- * size makes it matter by the same criterion (M s^2 > D^4 crit f, or BH
- * s^2 > theta f D^2, f = ewald_accuracy).  Their monopoles are already
- * inside the tabulated correction, so only the quadrupoles are added; a
- * node straddling the half-box planes is covered the same way. */
-__device__ __forceinline__ void image_quads(Acc32 &a, const float4 q0,
-		float qyz, float m, float s, float dx, float dy, float dz, float crit,
-		const Walk32Params &w)
-{
-	const float ex = dx > 0.f ? -w.box_f : w.box_f;
-	const float ey = dy > 0.f ? -w.box_f : w.box_f;
-	const float ez = dz > 0.f ? -w.box_f : w.box_f;
-	/* nearest other image: no closer than box - max |d_k| */
-	const float dmin = w.box_f - fmaxf(fabsf(dx), fmaxf(fabsf(dy), fabsf(dz)));
-	const float lhs = w.use_bh ? s * s : m * s * s;
-	auto needed = [&](float D2) {
-		return w.use_bh ? lhs > (w.theta_bh * w.ewald_acc) * D2
-				: lhs > (D2 * (crit * w.ewald_acc)) * D2;
-	};
-	if (!needed(dmin * dmin))
-		return;
-#pragma unroll 1
-	for (int n = 1; n < 8; n++) {
-		const float ix = (n & 1) ? dx + ex : dx;
-		const float iy = (n & 2) ? dy + ey : dy;
-		const float iz = (n & 4) ? dz + ez : dz;
-		const float D2 = fmaf(ix, ix, fmaf(iy, iy, iz * iz));
-		if (needed(D2))
-			quad32(a, q0, qyz, ix, iy, iz, w);
-	}
+		return true;
+	const float dimg = 1.f - fmaxf(fabsf(rx), fmaxf(fabsf(ry), fabsf(rz)))
+			* w.inv_box_f;
+	const float di2 = 1.f / (dimg * dimg);
+	const float K = fmaf(24.f * di2 * di2, 1.f / dimg, 60.f);
+	const float sb = s * w.inv_box_f;
+	return m * (sb * sb * sb) * K * (w.inv_box_f * w.inv_box_f)
+			<= crit * w.ewald_acc;
 }
 
 __global__ void __launch_bounds__(64) k_walk32(int nact,
 		const int *__restrict__ active, const int4 *__restrict__ p32,
 		const double *__restrict__ last_acc, TopNodes T,
 		const int4 *__restrict__ a32, const int4 *__restrict__ b32,
-		const int4 *__restrict__ nc, const float *__restrict__ tab,
+		const int4 *__restrict__ nc, const float4 *__restrict__ tab,
 		Walk32Params w, double *__restrict__ ax, double *__restrict__ ay,
 		double *__restrict__ az, double *__restrict__ gx,
 		double *__restrict__ gy, double *__restrict__ gz,
@@ -402,10 +465,10 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 				const float qyz = tq32[2 * j + 1].x;
 				quad32(a, q0, qyz, rx, ry, rz, w);
 				if (w.periodic)
-					image_quads(a, q0, qyz, tm, ts, rx, ry, rz, crit, w);
+					ewald16q(a, tab, tm, q0, qyz, rx, ry, rz, w);
+			} else if (w.periodic) {
+				ewald16(a, tab, tm, rx, ry, rz, w);
 			}
-			if (w.periodic)
-				ewald32(a, tab, tm, rx, ry, rz, w);
 			tick(a, every);
 			continue;
 		}
@@ -418,7 +481,7 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 				const float m = __int_as_float(pq.w);
 				newton32(a, m, dx, dy, dz, w);
 				if (w.periodic && (dx != 0.f || dy != 0.f || dz != 0.f))
-					ewald32(a, tab, m, dx, dy, dz, w);
+					ewald16(a, tab, m, dx, dy, dz, w);
 				tick(a, every);
 			}
 			continue;
@@ -433,32 +496,51 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 			const bool pending = node >= ew_until;
 			if (c.x < 0) {
 				const int f = -c.x - 1, l = f + c.y;
-				bool per_particle = false;
+				const int4 A = a32[node];
+				bool com_frame = false;      /* Ewald taken at the CoM */
 				if (pending) {
-					/* the bundle's correction at its CoM when every lane
-					 * allows it, else per particle */
-					const int4 A = a32[node];
+					/* the bundle's correction at its CoM to second order when
+					 * every lane allows it, else per particle */
 					const int4 B = b32[node];
 					const float s = __int_as_float(B.w),
 							m = __int_as_float(A.w);
 					const float rx = fdiff(A.x, me.x), ry = fdiff(A.y, me.y),
 							rz = fdiff(A.z, me.z);
 					const bool ok = ewald_ok(m, s, rx, ry, rz, fdiff(B.x, me.x),
-							fdiff(B.y, me.y), fdiff(B.z, me.z), crit, w);
-					if (__all(ok))
-						ewald32(a, tab, m, rx, ry, rz, w);
-					else
-						per_particle = true;
+							fdiff(B.y, me.y), fdiff(B.z, me.z), crit, true, w);
+					com_frame = __all(ok);
+					if (com_frame) {
+						if (w.quad)
+							ewald16q(a, tab, m, q32[2 * node],
+									q32[2 * node + 1].x, rx, ry, rz, w);
+						else
+							ewald16(a, tab, m, rx, ry, rz, w);
+					}
 				}
-				for (int q = f; q < l; q++) {
-					const int4 pq = p32[q];
-					const float dx = fdiff(pq.x, me.x), dy = fdiff(pq.y, me.y),
-							dz = fdiff(pq.z, me.z);
-					const float m = __int_as_float(pq.w);
-					newton32(a, m, dx, dy, dz, w);
-					if (per_particle && (dx != 0.f || dy != 0.f || dz != 0.f))
-						ewald32(a, tab, m, dx, dy, dz, w);
-					tick(a, every);
+				if (com_frame) {
+					/* Newton in the CoM's periodic frame, consistent with the
+					 * correction taken there: d = (A - me) + (q - A) */
+					const float cx = fdiff(A.x, me.x), cy = fdiff(A.y, me.y),
+							cz = fdiff(A.z, me.z);
+					for (int q = f; q < l; q++) {
+						const int4 pq = p32[q];
+						const float dx = cx + fdiff(pq.x, A.x),
+								dy = cy + fdiff(pq.y, A.y),
+								dz = cz + fdiff(pq.z, A.z);
+						newton32(a, __int_as_float(pq.w), dx, dy, dz, w);
+						tick(a, every);
+					}
+				} else {
+					for (int q = f; q < l; q++) {
+						const int4 pq = p32[q];
+						const float dx = fdiff(pq.x, me.x),
+								dy = fdiff(pq.y, me.y), dz = fdiff(pq.z, me.z);
+						const float m = __int_as_float(pq.w);
+						newton32(a, m, dx, dy, dz, w);
+						if (pending && (dx != 0.f || dy != 0.f || dz != 0.f))
+							ewald16(a, tab, m, dx, dy, dz, w);
+						tick(a, every);
+					}
 				}
 				node++;
 				continue;
@@ -486,17 +568,21 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 					const float qyz = q32[2 * node + 1].x;
 					quad32(a, q0, qyz, rx, ry, rz, w);
 					if (pending)
-						image_quads(a, q0, qyz, m, s, rx, ry, rz, crit, w);
+						ewald16q(a, tab, m, q0, qyz, rx, ry, rz, w);
+				} else if (pending) {
+					ewald16(a, tab, m, rx, ry, rz, w);
 				}
-				if (pending)
-					ewald32(a, tab, m, rx, ry, rz, w);
 				tick(a, every);
 				node += c.x;
 				continue;
 			}
 			if (pending && __all(ewald_ok(m, s, rx, ry, rz, sx, sy, sz, crit,
-					w))) {
-				ewald32(a, tab, m, rx, ry, rz, w);
+					false, w))) {
+				if (w.quad)
+					ewald16q(a, tab, m, q32[2 * node], q32[2 * node + 1].x, rx,
+							ry, rz, w);
+				else
+					ewald16(a, tab, m, rx, ry, rz, w);
 				ew_until = node + c.x;
 			}
 			node++;
@@ -589,13 +675,25 @@ int tdv_walk32(tdv_context *c, int use_bh)
 		hipLaunchKernelGGL(k_prep32_nodes, dim3(tdv_blocks(nn, 256)),
 				dim3(256), 0, c->stream, nn, c->node_a, c->node_b, o[0], o[1],
 				o[2], S, mscale, c->a32, c->b32);
-	if (c->par.periodic && !c->ewald32) {
-		TDV_HIP(c, hipMalloc((void **)&c->ewald32,
-				3 * (size_t)TDV_NE3 * sizeof(float)));
-		/* table / Box^2 (already in c->ewald), times the inverse mass and
-		 * length scaling: a = (G m S^2) F / (Box^2 S^2) ... / G */
-		hipLaunchKernelGGL(k_prep32_ewald, dim3(tdv_blocks(3 * TDV_NE3, 256)),
-				dim3(256), 0, c->stream, c->ewald, c->ewald32, 1.0 / (S * S));
+	if (c->par.periodic && !c->ewald16_ok) {
+		if (!c->have_ewald)
+			return tdv_fail(c, TDV_ERR_ARG, "FP32 walk without an Ewald table");
+		if (!c->ewald_s) {
+			TDV_HIP(c, hipMalloc((void **)&c->ewald_s,
+					10 * (size_t)TDV_NE3 * sizeof(double)));
+			TDV_TRY(tdv_ewald_derivs_device(c, c->ewald_s));
+		}
+		if (!c->ewald16)
+			TDV_HIP(c, hipMalloc((void **)&c->ewald16,
+					4 * (size_t)TDV_NE3 * sizeof(float4)));
+		/* E: c->ewald is F / Box^2; m' = G m S^2, so E / S^2.  S: per unit
+		 * box, / Box^4 for lengths, / 6 (T -> Q), / S^2 (T'' = T' / S^2). */
+		const double box = c->par.boxsize;
+		hipLaunchKernelGGL(k_prep32_ewald16, dim3(tdv_blocks(TDV_NE3, 256)),
+				dim3(256), 0, c->stream, c->ewald, c->ewald_s, 1.0 / (S * S),
+				1.0 / (6.0 * box * box * box * box * S * S), c->ewald16);
+		TDV_HIP(c, hipGetLastError());
+		c->ewald16_ok = 1;
 	}
 	Walk32Params w;
 	w.ntop = c->ntop;
@@ -614,12 +712,18 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	w.ewald_size = (float)(c->par.ewald_size * c->par.boxsize * S);
 	w.ewald_acc = (float)c->par.ewald_accuracy;
 	w.u_scale = (float)(2.0 * TDV_NE / 4294967296.0);
+	w.inv_box_f = 1.f / w.box_f;
+	w.inv_s2 = (float)(1.0 / (S * S));
+	{
+		const double hb = c->par.periodic ? c->par.boxsize / (2.0 * TDV_NE) : 0;
+		w.tri_corr = (float)(3.0 * hb * hb);
+	}
 	w.carry_every = c->par.fp32_carry > 0 ? c->par.fp32_carry : 16;
 	w.quad = quad ? 1 : 0;
 	w.crit3 = c->par.fp32_criterion == 3;
 	hipLaunchKernelGGL(k_walk32, dim3(tdv_blocks((size_t)c->nactive, 64)),
 			dim3(64), 0, c->stream, c->nactive, c->active, c->p32, c->last_acc,
-			c->top, c->a32, c->b32, c->node_c, c->ewald32, w, c->acc[0],
+			c->top, c->a32, c->b32, c->node_c, c->ewald16, w, c->acc[0],
 			c->acc[1], c->acc[2], c->gacc[0], c->gacc[1], c->gacc[2], c->cost,
 			c->d_count, c->top32, c->q32, c->tq32);
 	return tdv_check(c, hipGetLastError(), "k_walk32");

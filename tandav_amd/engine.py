@@ -80,6 +80,7 @@ def load_library(path=LIB_PATH):
     L.tdv_tree_acceleration.argtypes = [_ctx, ctypes.c_int]
     L.tdv_compute_acceleration.argtypes = [_ctx, ctypes.c_int]
     L.tdv_forcetest.argtypes = [_ctx, ctypes.c_int, _vp, _dp]
+    L.tdv_forcetest_exact.argtypes = [_ctx, ctypes.c_int, _vp, _dp]
     L.tdv_peano_keys.argtypes = [_ctx, _sz, _dp, _dp, _dp, _dp,
                                  ctypes.c_double, ctypes.c_int, _vp]
     L.tdv_sort_keys.argtypes = [_ctx, _sz, _vp, _vp]
@@ -101,6 +102,7 @@ def load_library(path=LIB_PATH):
                                        ctypes.POINTER(_dp), _dp]
     L.tdv_set_active.argtypes = [_ctx, ctypes.c_int, _vp]
     L.tdv_set_tree_moments.argtypes = [_ctx, _vp, _sz, _vp, ctypes.c_int]
+    L.tdv_get_ewald_derivs.argtypes = [_ctx, _dp]
     L.tdv_num_particles.argtypes = [_ctx]
     L.tdv_num_particles.restype = _sz
     _lib = L
@@ -274,6 +276,16 @@ class Engine:
         self._ok(self.lib.tdv_compute_acceleration(self._c, int(prepare_step)),
                  "tdv_compute_acceleration")
 
+    def forcetest_exact(self, idx):
+        """forcetest with every pair's Ewald correction summed exactly
+        instead of interpolated from the table."""
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        out = np.zeros((idx.size, 3))
+        self._ok(self.lib.tdv_forcetest_exact(self._c, idx.size,
+                                              idx.ctypes.data_as(_vp), _d(out)),
+                 "tdv_forcetest_exact")
+        return out
+
     def forcetest(self, idx):
         """Gravity_Forcetest (Gravity/forcetest.c:24-174): FP64 direct sum
         (+ Ewald correction per pair when periodic) for particles idx in the
@@ -374,6 +386,15 @@ class Engine:
         self._ok(self.lib.tdv_get_ewald_table(self._c, _d(t)),
                  "tdv_get_ewald_table")
         return t
+
+    def ewald_derivs(self):
+        """Second derivatives d_b d_c E_a of the Ewald correction at the
+        table nodes, (10, 65, 65, 65), per unit box, components
+        xxx yyy zzz xxy xxz xyy yyz xzz yzz xyz."""
+        s = np.zeros((10, 65, 65, 65))
+        self._ok(self.lib.tdv_get_ewald_derivs(self._c, _d(s)),
+                 "tdv_get_ewald_derivs")
+        return s
 
     def set_ewald_table(self, t):
         t = np.ascontiguousarray(t, dtype=np.float64)

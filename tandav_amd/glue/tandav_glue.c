@@ -350,12 +350,43 @@ static void download_forces(tdv_context *c, const bool last_acc)
 			last_acc ? P.Last_Acc_Mag : NULL, NULL, NULL, NULL, NULL));
 }
 
+/* check_total_momentum (Gravity/tree_accel.c:442-469), a host diagnostic
+ * over the host's P.Vel (the kicks run on the host): sum m v of the type-1
+ * particles into running sums that, as in Tandav, are never reset, and the
+ * relative change of their norm since the last call. */
+static double Mom_Px = 0, Mom_Py = 0, Mom_Pz = 0, Mom_Last = 0;
+
+static void check_total_momentum(const bool show_change)
+{
+	const int first = Task.Npart[0], last = Task.Npart[0] + Task.Npart[1];
+
+	#pragma omp for reduction(+: Mom_Px, Mom_Py, Mom_Pz)
+	for (int i = first; i < last; i++) {
+		Mom_Px += P.Mass[i] * P.Vel[0][i];
+		Mom_Py += P.Mass[i] * P.Vel[1][i];
+		Mom_Pz += P.Mass[i] * P.Vel[2][i];
+	}
+
+	const double ptotal = sqrt(Mom_Px * Mom_Px + Mom_Py * Mom_Py
+			+ Mom_Pz * Mom_Pz);
+	if (show_change)
+		rprintf("Total change in momentum due to gravity: %g \n",
+				(ptotal - Mom_Last) / Mom_Last);
+
+	#pragma omp barrier
+	#pragma omp single
+	Mom_Last = ptotal;
+}
+
 /* Gravity_Tree_Acceleration (Gravity/tree_accel.c:32-91), which also runs
  * Gravity_Tree_Periodic (tree_accel.c:84): the walk of the active particles
  * with the BH (Sig.Use_BH_Criterion) or relative criterion, then the Ewald
  * walk on periodic runs. */
 void Gravity_Tree_Acceleration(void)
 {
+	rprintf("Tree acceleration ");
+	check_total_momentum(false);
+
 	#pragma omp single
 	{
 	tdv_context *c = context();
@@ -363,6 +394,9 @@ void Gravity_Tree_Acceleration(void)
 	TDV_CHECK(tdv_tree_acceleration(c, Sig.Use_BH_Criterion ? 1 : 0));
 	download_forces(c, false);
 	} // omp single
+
+	rprintf(" done \n");
+	check_total_momentum(true);
 }
 
 /* Gravity_Tree_Periodic (Gravity/tree_periodic.c:23-72): the engine runs
@@ -509,12 +543,25 @@ void Compute_Acceleration(void)
 	if (Sig.Tree_Update)
 		Gravity_Tree_Build();
 
+	/* the momentum check brackets each walk (one, or BH + relative on the
+	 * prepare step); velocities do not change in between */
+	const int nwalks = Sig.Prepare_Step ? 2 : 1;
+	for (int w = 0; w < nwalks; w++) {
+		rprintf("Tree acceleration ");
+		check_total_momentum(false);
+	}
+
 	#pragma omp single
 	{
 	tdv_context *c = context();
 	TDV_CHECK(tdv_compute_acceleration(c, Sig.Prepare_Step ? 1 : 0));
 	download_forces(c, true);
 	} // omp single
+
+	for (int w = 0; w < nwalks; w++) {
+		rprintf(" done \n");
+		check_total_momentum(true);
+	}
 
 #ifdef GRAVITY_FORCETEST
 	Gravity_Forcetest();

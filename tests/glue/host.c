@@ -24,6 +24,15 @@ int NActive_Particles = 0;
 double Host_Theta_BH = 0.1, Host_Alpha_Rel = 0.005;
 
 static int Threads = 4;
+static bool Host_Verbose = false;
+
+/* rprintf output (Task.Is_Master) on or off */
+void host_set_verbose(int on)
+{
+	Host_Verbose = on != 0;
+	#pragma omp parallel num_threads(Threads)
+	Task.Is_Master = Host_Verbose;
+}
 static int Max_Active_Bin = 0;
 static size_t N = 0;
 
@@ -55,7 +64,9 @@ static void set_task(void)
 	#pragma omp parallel num_threads(Threads)
 	{
 	Task.Npart_Total = (int)N;
-	Task.Is_Master = false;
+	memset(Task.Npart, 0, sizeof(Task.Npart));
+	Task.Npart[1] = (int)N;           /* every particle is type 1 */
+	Task.Is_Master = Host_Verbose;
 	Task.Thread_ID = omp_get_thread_num();
 	Task.Seed[0] = 11;
 	Task.Seed[1] = 22;

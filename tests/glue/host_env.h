@@ -126,6 +126,7 @@ extern struct Local_Task_Properties {
 	int Thread_ID;
 	bool Is_Master;
 	int Npart_Total;
+	int Npart[NPARTYPE];
 	unsigned short Seed[3];
 } Task;
 #pragma omp threadprivate(Task)

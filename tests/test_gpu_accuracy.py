@@ -101,3 +101,88 @@ def test_fp32_quadrupoles_reduce_the_error(periodic, table):
     r1, _ = _fp32_error(pos, m, *args, order=1)
     assert np.percentile(r2, 99) < 0.7 * np.percentile(r1, 99)
     assert np.median(r2) < 0.7 * np.median(r1)
+
+
+def test_ewald_second_derivatives(table):
+    """The walk's second-order Ewald table against second differences of the
+    oracle's FP64 correction table (grid step 1/128 box), and the symmetry
+    and tracelessness the correction's curvature must have."""
+    e = Engine(periodic=True, softening=1e-3)
+    S = e.ewald_derivs()
+    E = -table[:3]                       # the correction on the octant
+    h = 0.5 / 64
+    sl = (slice(2, 62),) * 3             # interior nodes, away from the faces
+    comp = {"xxx": (0, 0), "yyy": (1, 1), "zzz": (2, 2), "xxy": (1, 0),
+            "xxz": (2, 0), "xyy": (0, 1), "yyz": (2, 1), "xzz": (0, 2),
+            "yzz": (1, 2)}                # name: (E component, axis)
+    names = ["xxx", "yyy", "zzz", "xxy", "xxz", "xyy", "yyz", "xzz", "yzz",
+             "xyz"]
+    scale = np.abs(S[:, 2:62, 2:62, 2:62]).max()
+    for q, name in enumerate(names[:9]):
+        c, ax = comp[name]
+        fd = (np.roll(E[c], -1, ax) - 2 * E[c] + np.roll(E[c], 1, ax)) / h**2
+        # O(h^2) differences of the table against the fine-step derivative
+        assert np.abs(fd[sl] - S[q][sl]).max() < 0.02 * scale, name
+    fd = (np.roll(np.roll(E[0], -1, 1), -1, 2) - np.roll(np.roll(E[0], -1, 1),
+          1, 2) - np.roll(np.roll(E[0], 1, 1), -1, 2)
+          + np.roll(np.roll(E[0], 1, 1), 1, 2)) / (4 * h * h)
+    assert np.abs(fd[sl] - S[9][sl]).max() < 0.02 * scale
+    # traceless in the last two indices: S_axx + S_ayy + S_azz = 0
+    for a, (i1, i2, i3) in enumerate([(0, 5, 7), (3, 1, 8), (4, 6, 2)]):
+        tr = S[i1] + S[i2] + S[i3]
+        assert np.abs(tr).max() < 1e-4 * scale, a
+    # odd in the axes that appear an odd number of times: zero on those faces
+    assert np.abs(S[0][0]).max() < 1e-9 * scale       # xxx at x = 0
+    assert np.abs(S[9][:, 0]).max() < 1e-9 * scale    # xyz at y = 0
+    # E''(d) grows from zero at the origin (cubic lattice)
+    assert np.abs(S[:, 0, 0, 0]).max() < 1e-6 * scale
+
+
+def test_forcetest_exact_on_table_nodes(table):
+    """The exact-Ewald direct sum against the oracle's table: separations
+    placed on table nodes, where trilinear interpolation returns the node
+    value, so both sums must agree to rounding."""
+    g = np.random.Generator(np.random.Philox(12))
+    h = 1.0 / 128
+    base = np.array([0.3, 0.4, 0.45])
+    steps = g.integers(-60, 61, (40, 3))
+    steps = steps[np.abs(steps).sum(1) > 0]
+    pos = np.vstack([base, np.mod(base + steps * h, 1.0)])
+    m = np.full(len(pos), 1.0 / len(pos))
+    e = Engine(periodic=True, softening=1e-4, ewald_table=table)
+    e.set_particles(pos, m)
+    e.setup_domain()
+    e.domain_decomposition()
+    ids = e.get("id")["id"]
+    i0 = int(np.flatnonzero(ids == 0)[0])          # the base particle
+    got = e.forcetest_exact([i0])
+    want = e.forcetest([i0])
+    np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-12)
+
+
+def test_fp32_walk_against_exact_ewald():
+    """The FP32 group walk at the benchmark's settings (alpha 0.01 with the
+    next-order criterion, quadrupoles, second-order Ewald) against FP64
+    direct summation with exact Ewald sums: p99 |da|/|a| <= 2e-3, and far
+    below the FP64 walk's error on the same box."""
+    pos, _, m = ics.zeldovich_box(48, seed=7)
+    soft = 1 / 1920.
+    e64 = Engine(periodic=True, softening=soft, alpha_rel=0.005)
+    e32 = Engine(periodic=True, softening=soft, alpha_rel=0.01,
+                 precision="fp32", compat=False, ewald_size=0.125,
+                 ewald_accuracy=1.0, fp32_order=2, fp32_criterion=3)
+    out = []
+    for e in (e64, e32):
+        e.set_particles(pos, m)
+        e.setup_domain()
+        e.domain_decomposition()
+        e.compute_acceleration(True)
+        e.domain_decomposition()
+        e.compute_acceleration(False)
+        out.append(e.get("acc")["acc"])
+    g = np.random.Generator(np.random.Philox(5))
+    idx = np.sort(g.choice(len(m), 256, replace=False))
+    ref = e64.forcetest_exact(idx)
+    r32, r64 = rel_err(out[1][idx], ref), rel_err(out[0][idx], ref)
+    assert np.percentile(r32, 99) <= 2e-3
+    assert np.percentile(r32, 99) < 0.5 * np.percentile(r64, 99)
