@@ -96,6 +96,28 @@ def run_gpu(pos, m, soft, w, steps, warmup, precision, f32):
     return e, (t1 - t0) / steps, {k: v / steps for k, v in stage.items()}
 
 
+def measured_traffic(kernel, workload, f32):
+    """HBM bytes per launch of the dominant kernel from the committed PMC
+    measurement (FETCH_SIZE + WRITE_SIZE, each scaled by a plain copy's
+    reading), when it was taken on this workload, these settings and this
+    kernel source; else None."""
+    import hashlib
+    path = os.path.join(ROOT, "profiles", "walk32_c2_traffic.json")
+    src = os.path.join(ROOT, "tandav_amd", "csrc", "tdv_walk32.hip")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        with open(src, "rb") as f:
+            sha = hashlib.sha256(f.read()).hexdigest()
+    except OSError:
+        return None
+    same = all(abs(d["fp32"][k] - f32[k]) < 1e-12 for k in d["fp32"])
+    if (d["kernel"] != kernel or d["workload"] != workload or not same
+            or d["walk32_source_sha256"] != sha):
+        return None
+    return d["traffic_bytes"]
+
+
 def errors(a, ref):
     r = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
     return {"median": float(np.median(r)), "p99": float(np.percentile(r, 99)),
@@ -161,7 +183,21 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp32", "fp64"], default=None)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device (one plain copy "
+                         "kernel, the reference point for the HBM byte "
+                         "counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        nb = args.copy_calibration << 20
+        src = torch.ones(nb // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": nb}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus %d is not "
               "supported (multi-GPU sharding is a design note only)"
@@ -227,7 +263,8 @@ def main():
     achieved = flops / (t_k * 1e-3)
     roof = {"bound": "valu", "kernel": kern, "achieved": achieved / 1e12,
             "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
-            "traffic": None, "avg_launch_ms": t_k,
+            "traffic": measured_traffic(kern, args.workload, f32),
+            "avg_launch_ms": t_k,
             "flops_per_launch": flops,
             "counting": "20 flops per monopole interaction, 60 per Ewald "
                         "correction; interactions counted on the device"}
