@@ -71,6 +71,8 @@ typedef struct {
 	int fp32_carry;        /* FP32 walk: interactions summed in FP32 before
 	                          the partial sums are carried into FP64
 	                          (0 = 16)                                        */
+	int fp32_waves;        /* FP32 walk: register budget for 6 or 8 waves per
+	                          SIMD (0 = the compiler's own allocation)        */
 } tdv_params;
 
 enum { TDV_WALK_FP64 = 0, TDV_WALK_FP32 = 1 };

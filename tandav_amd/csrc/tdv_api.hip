@@ -118,6 +118,7 @@ static void free_particles(tdv_context *c)
 	dfree(c->lohi);
 	dfree(c->top.ptop);
 	dfree(c->p32);                   /* FP32 walk records, sized by cap */
+	dfree(c->p32o);
 	c->cap = 0;
 }
 
@@ -258,6 +259,8 @@ int tdv_create(const tdv_params *p, tdv_context **out)
 		return TDV_ERR_ARG;
 	if (p->fp32_carry < 0 || p->fp32_carry > 4096)
 		return TDV_ERR_ARG;
+	if (p->fp32_waves != 0 && p->fp32_waves != 6 && p->fp32_waves != 8)
+		return TDV_ERR_ARG;
 	tdv_context *c = new (std::nothrow) tdv_context();
 	if (!c)
 		return TDV_ERR_NOMEM;
@@ -307,6 +310,7 @@ void tdv_destroy(tdv_context *c)
 	dfree(c->ewald16);
 	dfree(c->ewald_s);
 	dfree(c->p32);
+	dfree(c->p32o);
 	dfree(c->a32);
 	dfree(c->b32);
 	dfree(c->q32);

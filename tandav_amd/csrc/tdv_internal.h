@@ -96,6 +96,7 @@ struct tdv_context {
 
 	/* FP32 group walk: fixed-point records */
 	int4 *p32, *a32, *b32;
+	float4 *p32o;                    /* offset from the leaf's CoM, mass' */
 	size_t n32_cap;
 	float4 *q32;                     /* 2 per node: traceless quadrupole */
 	float4 *tq32;                    /* 2 per top node */

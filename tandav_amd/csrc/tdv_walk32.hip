@@ -43,7 +43,8 @@ struct Walk32Params {
 	float u_scale;                   /* fixed units -> table grid units */
 	float inv_box_f;                 /* 1 / box_f */
 	float inv_s2;                    /* 1 / S^2: quadrupoles -> table units */
-	float tri_corr;                  /* 3 (Box h)^2: trilinear error from S */
+	float tri_corr;                  /* trilinear error from S, table units */
+	float inv_s2_box4;               /* 1 / (S^2 Box^4) */
 	int carry_every;
 	int quad;                        /* accepted nodes add their quadrupole */
 	int crit3;                       /* open if M s^3 > r^5 |a| alpha / G */
@@ -107,29 +108,63 @@ __global__ void k_prep32_top(int ntop, const TopNodes T, double o0, double o1,
 			__float_as_int((float)m));
 }
 
-/* The walk's Ewald records, one per table node (4 float4 = 64 bytes, one
- * cache line per trilinear corner): {Ex, Ey, Ez, Sxxx}, {Syyy, Szzz, Sxxy,
- * Sxxz}, {Sxyy, Syyz, Sxzz, Syzz}, {Sxyz, 0, 0, 0}.  E = -F (the table's
- * sign rule: Ewald_Correction returns -F(|d|) on the positive octant) in
- * units where m' E is an acceleration (m' = G m S^2); S = d d E / 6 in the
- * units that T'' = T' / S^2 contracts to an acceleration (the factor 1/6
- * turns the traceless T into the second moment: Q:S/2 = T:S/6). */
+/* The walk's Ewald records, one per table node, 32 bytes: {Ex, Ey, Ez,
+ * (Sxxx, Syyy)}, {(Szzz, Sxxy), (Sxxz, Sxyy), (Syyz, Sxzz), (Syzz, Sxyz)},
+ * pairs of FP16 in one 32-bit word.  E = -F (the table's sign rule:
+ * Ewald_Correction returns -F(|d|) on the positive octant) in units where
+ * m' E is an acceleration (m' = G m S^2).  S = d d E / 6 per unit box: the
+ * second derivatives only enter second-order terms, where FP16's 5e-4
+ * relative precision is far below the terms' own size. */
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float pack_h2(double a, double b)
+{
+	half2_t h;
+	h.x = (_Float16)a;
+	h.y = (_Float16)b;
+	return __builtin_bit_cast(float, h);
+}
+
 __global__ void k_prep32_ewald16(const double *__restrict__ raw,
-		const double *__restrict__ s10, double e_scale, double s_scale,
+		const double *__restrict__ s10, double e_scale,
 		float4 *__restrict__ t16)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= TDV_NE3)
 		return;
-	float v[16];
-	for (int d = 0; d < 3; d++)
-		v[d] = (float)(-raw[(size_t)d * TDV_NE3 + i] * e_scale);
+	double S[10];
 	for (int q = 0; q < 10; q++)
-		v[3 + q] = (float)(s10[(size_t)q * TDV_NE3 + i] * s_scale);
-	v[13] = v[14] = v[15] = 0.f;
-	for (int r = 0; r < 4; r++)
-		t16[4 * i + r] = make_float4(v[4 * r], v[4 * r + 1], v[4 * r + 2],
-				v[4 * r + 3]);
+		S[q] = s10[(size_t)q * TDV_NE3 + i] / 6.0;
+	t16[2 * i] = make_float4((float)(-raw[i] * e_scale),
+			(float)(-raw[(size_t)TDV_NE3 + i] * e_scale),
+			(float)(-raw[2 * (size_t)TDV_NE3 + i] * e_scale),
+			pack_h2(S[0], S[1]));
+	t16[2 * i + 1] = make_float4(pack_h2(S[2], S[3]), pack_h2(S[4], S[5]),
+			pack_h2(S[6], S[7]), pack_h2(S[8], S[9]));
+}
+
+/* Every particle of a bundle: its offset from the bundle's centre of mass
+ * in fixed units (an exact integer difference, rounded once to FP32) and
+ * its mass', so a group taking the bundle in the centre's frame adds one
+ * FP32 add per component.  Particles of small top nodes (no leaf) are not
+ * read from here. */
+__global__ void k_prep32_offsets(size_t n, const int4 *__restrict__ p32,
+		const int *__restrict__ leaf, const int4 *__restrict__ a32,
+		float4 *__restrict__ p32o)
+{
+	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const int4 p = p32[i];
+	const int j = leaf[i];
+	float4 o = make_float4(0.f, 0.f, 0.f, __int_as_float(p.w));
+	if (j >= 0) {
+		const int4 c = a32[j];
+		o.x = (float)(int)((unsigned)p.x - (unsigned)c.x);
+		o.y = (float)(int)((unsigned)p.y - (unsigned)c.y);
+		o.z = (float)(int)((unsigned)p.z - (unsigned)c.z);
+	}
+	p32o[i] = o;
 }
 
 /* Traceless second moment T = sum m (3 y y^T - |y|^2 I), y = x - CoM, of
@@ -219,6 +254,14 @@ __device__ __forceinline__ void carry(Acc32 &a)
 	a.since = 0;
 }
 
+/* after n interactions: carry when the partial sums have taken `every` */
+__device__ __forceinline__ void tickn(Acc32 &a, int n, int every)
+{
+	a.since += n;
+	if (a.since >= every)
+		carry(a);
+}
+
 __device__ __forceinline__ void tick(Acc32 &a, int every)
 {
 	if (++a.since >= every)
@@ -229,19 +272,22 @@ __device__ __forceinline__ void tick(Acc32 &a, int every)
 __device__ __forceinline__ void newton32(Acc32 &a, float m, float dx,
 		float dy, float dz, const Walk32Params &w)
 {
+	/* every lane takes every interaction of the group's list, so the count
+	 * is wave-uniform (it includes the particle itself, which adds nothing) */
+	a.cost++;
 	const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-	if (r2 == 0.f)
-		return;
-	const float ri = __frsqrt_rn(r2);
-	float fac = m * ri * ri * ri;
-	if (r2 < w.eps2) {
+	const float ri = __frsqrt_rn(r2);        /* inf at r = 0: not used */
+	float fac = m * ri * (ri * ri);
+	/* inside the softening length (rare; the particle itself, at r = 0,
+	 * lands here too and adds fac * 0): only when some lane needs it */
+	if (__any(r2 < w.eps2)) {
 		const float u2 = r2 * w.inv_eps2;
-		fac = m * (175.f - u2 * (294.f - u2 * 135.f)) * w.k_inner;
+		const float in = m * (175.f - u2 * (294.f - u2 * 135.f)) * w.k_inner;
+		fac = r2 < w.eps2 ? in : fac;
 	}
 	a.x = fmaf(fac, dx, a.x);
 	a.y = fmaf(fac, dy, a.y);
 	a.z = fmaf(fac, dz, a.z);
-	a.cost++;
 }
 
 /* Quadrupole term of a node seen at d (particle -> node CoM):
@@ -306,7 +352,7 @@ __device__ __forceinline__ void ewald16(Acc32 &a, const float4 *__restrict__ t,
 		const int di = n >> 2, dj = (n >> 1) & 1, dk = n & 1;
 		const float wt = (di ? c.fu : 1.f - c.fu) * (dj ? c.fv : 1.f - c.fv)
 				* (dk ? c.fq : 1.f - c.fq);
-		const float4 r0 = t[4 * (c.b + (di * TDV_NE1 + dj) * TDV_NE1 + dk)];
+		const float4 r0 = t[2 * (c.b + (di * TDV_NE1 + dj) * TDV_NE1 + dk)];
 		ex = fmaf(wt, r0.x, ex);
 		ey = fmaf(wt, r0.y, ey);
 		ez = fmaf(wt, r0.z, ez);
@@ -319,65 +365,65 @@ __device__ __forceinline__ void ewald16(Acc32 &a, const float4 *__restrict__ t,
 
 /* The correction of a node of mass m' and traceless second moment T'
  * {Txx, Tyy, Txy, Txz}, Tyz about its centre of mass at d, to second order:
- * m' E(d) + T'':S(d), T'' = T'/S^2 (S = d d E / 6 in table units).  S is
- * symmetric in its three indices and traceless in the last two, so the
- * trace part of the second moment adds nothing. */
+ * m' E(d) + Q:S(d) with Q:S = T:S/6 (S = d d E is symmetric in its three
+ * indices and traceless in the last two, so the trace part of the second
+ * moment adds nothing), and the trilinear interpolation's own second-order
+ * error, u(1-u) h^2 d_k^2 E / 2 per axis, taken off with the same S.  The
+ * reflection signs (S_abc is odd in each axis that appears an odd number of
+ * times) and the trilinear correction are folded into seven coefficients,
+ * so each corner adds three 7-term dot products. */
+__device__ __forceinline__ float h2lo(float v)
+{
+	return (float)__builtin_bit_cast(half2_t, v).x;
+}
+
+__device__ __forceinline__ float h2hi(float v)
+{
+	return (float)__builtin_bit_cast(half2_t, v).y;
+}
+
 __device__ __forceinline__ void ewald16q(Acc32 &a, const float4 *__restrict__ t,
 		float m, const float4 qa, float qyz, float dx, float dy, float dz,
 		const Walk32Params &w)
 {
 	const EwaldCell c = ewald_cell(dx, dy, dz, w);
-	float e[3] = { 0.f, 0.f, 0.f }, S[10];
-#pragma unroll
-	for (int q = 0; q < 10; q++)
-		S[q] = 0.f;
-#pragma unroll
+	/* T' (masses G S^2, lengths S) to table units: / (S^4 Box^4), in two
+	 * steps so no constant is subnormal */
+	const float tx = qa.x * w.inv_s2 * w.inv_s2_box4,
+			ty = qa.y * w.inv_s2 * w.inv_s2_box4;
+	const float tri = m * w.tri_corr;
+	const float cxx = fmaf(-tri, c.fu * (1.f - c.fu), tx);
+	const float cyy = fmaf(-tri, c.fv * (1.f - c.fv), ty);
+	const float czz = fmaf(-tri, c.fq * (1.f - c.fq), -(tx + ty));
+	const float t2 = 2.f * w.inv_s2 * w.inv_s2_box4;
+	const float cxy = qa.z * t2 * (c.sx * c.sy), cxz = qa.w * t2 * (c.sx * c.sz),
+			cyz = qyz * t2 * (c.sy * c.sz);
+	float ax = 0.f, ay = 0.f, az = 0.f;
+#pragma unroll 2
 	for (int n = 0; n < 8; n++) {
 		const int di = n >> 2, dj = (n >> 1) & 1, dk = n & 1;
 		const float wt = (di ? c.fu : 1.f - c.fu) * (dj ? c.fv : 1.f - c.fv)
 				* (dk ? c.fq : 1.f - c.fq);
-		const float4 *r = t + 4 * (c.b + (di * TDV_NE1 + dj) * TDV_NE1 + dk);
-		const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
-		e[0] = fmaf(wt, r0.x, e[0]);
-		e[1] = fmaf(wt, r0.y, e[1]);
-		e[2] = fmaf(wt, r0.z, e[2]);
-		S[0] = fmaf(wt, r0.w, S[0]);
-		S[1] = fmaf(wt, r1.x, S[1]);
-		S[2] = fmaf(wt, r1.y, S[2]);
-		S[3] = fmaf(wt, r1.z, S[3]);
-		S[4] = fmaf(wt, r1.w, S[4]);
-		S[5] = fmaf(wt, r2.x, S[5]);
-		S[6] = fmaf(wt, r2.y, S[6]);
-		S[7] = fmaf(wt, r2.z, S[7]);
-		S[8] = fmaf(wt, r2.w, S[8]);
-		S[9] = fmaf(wt, r3.x, S[9]);
+		const float4 *r = t + 2 * (c.b + (di * TDV_NE1 + dj) * TDV_NE1 + dk);
+		const float4 r0 = r[0], r1 = r[1];
+		const float Sxxx = h2lo(r0.w), Syyy = h2hi(r0.w), Szzz = h2lo(r1.x),
+				Sxxy = h2hi(r1.x), Sxxz = h2lo(r1.y), Sxyy = h2hi(r1.y),
+				Syyz = h2lo(r1.z), Sxzz = h2hi(r1.z), Syzz = h2lo(r1.w),
+				Sxyz = h2hi(r1.w);
+		float vx = m * r0.x, vy = m * r0.y, vz = m * r0.z;
+		vx = fmaf(cxx, Sxxx, fmaf(cyy, Sxyy, fmaf(czz, Sxzz, vx)));
+		vx = fmaf(cxy, Sxxy, fmaf(cxz, Sxxz, fmaf(cyz, Sxyz, vx)));
+		vy = fmaf(cxx, Sxxy, fmaf(cyy, Syyy, fmaf(czz, Syzz, vy)));
+		vy = fmaf(cxy, Sxyy, fmaf(cxz, Sxyz, fmaf(cyz, Syyz, vy)));
+		vz = fmaf(cxx, Sxxz, fmaf(cyy, Syyz, fmaf(czz, Szzz, vz)));
+		vz = fmaf(cxy, Sxyz, fmaf(cxz, Sxzz, fmaf(cyz, Syzz, vz)));
+		ax = fmaf(wt, vx, ax);
+		ay = fmaf(wt, vy, ay);
+		az = fmaf(wt, vz, az);
 	}
-	/* trilinear interpolation overstates E by u(1-u) h^2 d_k^2 E / 2 per
-	 * axis; S holds those second derivatives (times 1/6 in table units) */
-	{
-		const float gu = c.fu * (1.f - c.fu), gv = c.fv * (1.f - c.fv),
-				gq = c.fq * (1.f - c.fq);
-		e[0] -= w.tri_corr * fmaf(gu, S[0], fmaf(gv, S[5], gq * S[7]));
-		e[1] -= w.tri_corr * fmaf(gu, S[3], fmaf(gv, S[1], gq * S[8]));
-		e[2] -= w.tri_corr * fmaf(gu, S[4], fmaf(gv, S[6], gq * S[2]));
-	}
-	/* reflect: S_abc is odd in an axis that appears an odd number of times */
-	const float sxxx = c.sx * S[0], syyy = c.sy * S[1], szzz = c.sz * S[2],
-			sxxy = c.sy * S[3], sxxz = c.sz * S[4], sxyy = c.sx * S[5],
-			syyz = c.sz * S[6], sxzz = c.sx * S[7], syzz = c.sy * S[8],
-			sxyz = c.sx * c.sy * c.sz * S[9];
-	const float txx = qa.x * w.inv_s2, tyy = qa.y * w.inv_s2,
-			tzz = -(txx + tyy), txy = 2.f * qa.z * w.inv_s2,
-			txz = 2.f * qa.w * w.inv_s2, tyz = 2.f * qyz * w.inv_s2;
-	const float qx = txx * sxxx + tyy * sxyy + tzz * sxzz + txy * sxxy
-			+ txz * sxxz + tyz * sxyz;
-	const float qy = txx * sxxy + tyy * syyy + tzz * syzz + txy * sxyy
-			+ txz * sxyz + tyz * syyz;
-	const float qz = txx * sxxz + tyy * syyz + tzz * szzz + txy * sxyz
-			+ txz * sxzz + tyz * syzz;
-	a.x += fmaf(m * c.sx, e[0], qx);
-	a.y += fmaf(m * c.sy, e[1], qy);
-	a.z += fmaf(m * c.sz, e[2], qz);
+	a.x = fmaf(c.sx, ax, a.x);
+	a.y = fmaf(c.sy, ay, a.y);
+	a.z = fmaf(c.sz, az, a.z);
 	a.ewald++;
 }
 
@@ -425,7 +471,7 @@ __device__ __forceinline__ bool ewald_ok(float m, float s, float rx, float ry,
 			<= crit * w.ewald_acc;
 }
 
-__global__ void __launch_bounds__(64) k_walk32(int nact,
+__device__ __forceinline__ void walk32_body(int nact,
 		const int *__restrict__ active, const int4 *__restrict__ p32,
 		const double *__restrict__ last_acc, TopNodes T,
 		const int4 *__restrict__ a32, const int4 *__restrict__ b32,
@@ -435,7 +481,7 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 		double *__restrict__ gy, double *__restrict__ gz,
 		double *__restrict__ cost, unsigned long long *counter,
 		const int4 *__restrict__ top32, const float4 *__restrict__ q32,
-		const float4 *__restrict__ tq32)
+		const float4 *__restrict__ tq32, const float4 *__restrict__ p32o)
 {
 	const int gid = blockIdx.x * 64 + threadIdx.x;
 	const bool live = gid < nact;
@@ -480,7 +526,7 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 						dz = fdiff(pq.z, me.z);
 				const float m = __int_as_float(pq.w);
 				newton32(a, m, dx, dy, dz, w);
-				if (w.periodic && (dx != 0.f || dy != 0.f || dz != 0.f))
+				if (w.periodic)           /* E(0) = 0: the self term adds 0 */
 					ewald16(a, tab, m, dx, dy, dz, w);
 				tick(a, every);
 			}
@@ -497,7 +543,11 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 			if (c.x < 0) {
 				const int f = -c.x - 1, l = f + c.y;
 				const int4 A = a32[node];
-				bool com_frame = false;      /* Ewald taken at the CoM */
+				/* Newton in the CoM's periodic frame unless the correction
+				 * is taken per particle: a correction taken higher up came
+				 * from a node that does not straddle the half-box planes,
+				 * so its particles share the CoM's image for every lane */
+				bool com_frame = !pending;
 				if (pending) {
 					/* the bundle's correction at its CoM to second order when
 					 * every lane allows it, else per particle */
@@ -522,22 +572,33 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 					 * correction taken there: d = (A - me) + (q - A) */
 					const float cx = fdiff(A.x, me.x), cy = fdiff(A.y, me.y),
 							cz = fdiff(A.z, me.z);
-					for (int q = f; q < l; q++) {
-						const int4 pq = p32[q];
-						const float dx = cx + fdiff(pq.x, A.x),
-								dy = cy + fdiff(pq.y, A.y),
-								dz = cz + fdiff(pq.z, A.z);
-						newton32(a, __int_as_float(pq.w), dx, dy, dz, w);
+					/* four particles per wide scalar load */
+					int q = f;
+					for (; q + 4 <= l; q += 4) {
+						const float4 p0 = p32o[q], p1 = p32o[q + 1],
+								p2 = p32o[q + 2], p3 = p32o[q + 3];
+						newton32(a, p0.w, cx + p0.x, cy + p0.y, cz + p0.z, w);
+						newton32(a, p1.w, cx + p1.x, cy + p1.y, cz + p1.z, w);
+						newton32(a, p2.w, cx + p2.x, cy + p2.y, cz + p2.z, w);
+						newton32(a, p3.w, cx + p3.x, cy + p3.y, cz + p3.z, w);
+						tickn(a, 4, every);
+					}
+					for (; q < l; q++) {
+						const float4 p0 = p32o[q];
+						newton32(a, p0.w, cx + p0.x, cy + p0.y, cz + p0.z, w);
 						tick(a, every);
 					}
 				} else {
+					int4 pn = p32[f];
 					for (int q = f; q < l; q++) {
-						const int4 pq = p32[q];
+						const int4 pq = pn;
+						if (q + 1 < l)
+							pn = p32[q + 1];
 						const float dx = fdiff(pq.x, me.x),
 								dy = fdiff(pq.y, me.y), dz = fdiff(pq.z, me.z);
 						const float m = __int_as_float(pq.w);
 						newton32(a, m, dx, dy, dz, w);
-						if (pending && (dx != 0.f || dy != 0.f || dz != 0.f))
+						if (pending)          /* E(0) = 0 for the self term */
 							ewald16(a, tab, m, dx, dy, dz, w);
 						tick(a, every);
 					}
@@ -611,6 +672,39 @@ __global__ void __launch_bounds__(64) k_walk32(int nact,
 	}
 }
 
+/* The walk with a register budget for W waves per SIMD (0: the
+ * compiler's own); the autotuner picks W per workload. */
+#define TDV_WALK32_ARGS int nact, const int *__restrict__ active, \
+		const int4 *__restrict__ p32, const double *__restrict__ last_acc, \
+		TopNodes T, const int4 *__restrict__ a32, const int4 *__restrict__ b32, \
+		const int4 *__restrict__ nc, const float4 *__restrict__ tab, \
+		Walk32Params w, double *__restrict__ ax, double *__restrict__ ay, \
+		double *__restrict__ az, double *__restrict__ gx, \
+		double *__restrict__ gy, double *__restrict__ gz, \
+		double *__restrict__ cost, unsigned long long *counter, \
+		const int4 *__restrict__ top32, const float4 *__restrict__ q32, \
+		const float4 *__restrict__ tq32, const float4 *__restrict__ p32o
+#define TDV_WALK32_CALL walk32_body(nact, active, p32, last_acc, T, a32, b32, \
+		nc, tab, w, ax, ay, az, gx, gy, gz, cost, counter, top32, q32, tq32, \
+		p32o)
+
+__global__ void __launch_bounds__(64) k_walk32(TDV_WALK32_ARGS)
+{
+	TDV_WALK32_CALL;
+}
+
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
+k_walk32_w6(TDV_WALK32_ARGS)
+{
+	TDV_WALK32_CALL;
+}
+
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8)))
+k_walk32_w8(TDV_WALK32_ARGS)
+{
+	TDV_WALK32_CALL;
+}
+
 int tdv_walk32(tdv_context *c, int use_bh)
 {
 	if (!c->nactive)
@@ -623,8 +717,10 @@ int tdv_walk32(tdv_context *c, int use_bh)
 		c->par.periodic ? 0.0 : c->dom_origin[1],
 		c->par.periodic ? 0.0 : c->dom_origin[2] };
 	const double mscale = c->par.gravity * S * S;
-	if (!c->p32)
+	if (!c->p32) {
 		TDV_HIP(c, hipMalloc((void **)&c->p32, c->cap * sizeof(int4)));
+		TDV_HIP(c, hipMalloc((void **)&c->p32o, c->cap * sizeof(float4)));
+	}
 	if (c->n32_cap < c->node_cap) {
 		if (c->a32)
 			(void)hipFree(c->a32);
@@ -675,6 +771,8 @@ int tdv_walk32(tdv_context *c, int use_bh)
 		hipLaunchKernelGGL(k_prep32_nodes, dim3(tdv_blocks(nn, 256)),
 				dim3(256), 0, c->stream, nn, c->node_a, c->node_b, o[0], o[1],
 				o[2], S, mscale, c->a32, c->b32);
+	hipLaunchKernelGGL(k_prep32_offsets, dim3(tdv_blocks(n, 256)), dim3(256),
+			0, c->stream, n, c->p32, c->tree_parent, c->a32, c->p32o);
 	if (c->par.periodic && !c->ewald16_ok) {
 		if (!c->have_ewald)
 			return tdv_fail(c, TDV_ERR_ARG, "FP32 walk without an Ewald table");
@@ -685,13 +783,11 @@ int tdv_walk32(tdv_context *c, int use_bh)
 		}
 		if (!c->ewald16)
 			TDV_HIP(c, hipMalloc((void **)&c->ewald16,
-					4 * (size_t)TDV_NE3 * sizeof(float4)));
-		/* E: c->ewald is F / Box^2; m' = G m S^2, so E / S^2.  S: per unit
-		 * box, / Box^4 for lengths, / 6 (T -> Q), / S^2 (T'' = T' / S^2). */
-		const double box = c->par.boxsize;
+					2 * (size_t)TDV_NE3 * sizeof(float4)));
+		/* E: c->ewald is F / Box^2; m' = G m S^2, so E / S^2 */
 		hipLaunchKernelGGL(k_prep32_ewald16, dim3(tdv_blocks(TDV_NE3, 256)),
 				dim3(256), 0, c->stream, c->ewald, c->ewald_s, 1.0 / (S * S),
-				1.0 / (6.0 * box * box * box * box * S * S), c->ewald16);
+				c->ewald16);
 		TDV_HIP(c, hipGetLastError());
 		c->ewald16_ok = 1;
 	}
@@ -702,7 +798,12 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	w.use_bh = use_bh;
 	w.theta_bh = (float)c->par.theta_bh;
 	w.alpha_rel = (float)c->par.alpha_rel;
-	const double ef = c->Epsilon * S;          /* negative, fixed units */
+	/* negative (the reference's K1 length), fixed units; at least half a
+	 * unit, so the particle itself (r = 0) always takes the finite inner
+	 * kernel */
+	double ef = c->Epsilon * S;
+	if (ef * ef < 0.5)
+		ef = -sqrt(0.5);
 	w.eps2 = (float)(ef * ef);
 	w.inv_eps2 = (float)(1.0 / (ef * ef));
 	w.k_inner = (float)(1.0 / (16.0 * ef * ef * ef));
@@ -715,16 +816,22 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	w.inv_box_f = 1.f / w.box_f;
 	w.inv_s2 = (float)(1.0 / (S * S));
 	{
-		const double hb = c->par.periodic ? c->par.boxsize / (2.0 * TDV_NE) : 0;
-		w.tri_corr = (float)(3.0 * hb * hb);
+		/* u(1-u) h^2 (d_k^2 E) / 2 in E's table units (E / (Box^2 S^2)):
+		 * 3 h_u^2 S_half / (Box^2 S^2) with h_u = 1/128, S_half = S / 6 */
+		const double box = c->par.periodic ? c->par.boxsize : 1.0;
+		const double hu = 1.0 / (2.0 * TDV_NE);
+		w.tri_corr = (float)(3.0 * hu * hu / (box * box * S * S));
+		w.inv_s2_box4 = (float)(1.0 / (S * S * box * box * box * box));
 	}
 	w.carry_every = c->par.fp32_carry > 0 ? c->par.fp32_carry : 16;
 	w.quad = quad ? 1 : 0;
 	w.crit3 = c->par.fp32_criterion == 3;
-	hipLaunchKernelGGL(k_walk32, dim3(tdv_blocks((size_t)c->nactive, 64)),
+	auto kern = c->par.fp32_waves == 6 ? k_walk32_w6
+			: c->par.fp32_waves == 8 ? k_walk32_w8 : k_walk32;
+	hipLaunchKernelGGL(kern, dim3(tdv_blocks((size_t)c->nactive, 64)),
 			dim3(64), 0, c->stream, c->nactive, c->active, c->p32, c->last_acc,
 			c->top, c->a32, c->b32, c->node_c, c->ewald16, w, c->acc[0],
 			c->acc[1], c->acc[2], c->gacc[0], c->gacc[1], c->gacc[2], c->cost,
-			c->d_count, c->top32, c->q32, c->tq32);
+			c->d_count, c->top32, c->q32, c->tq32, c->p32o);
 	return tdv_check(c, hipGetLastError(), "k_walk32");
 }

@@ -38,7 +38,8 @@ class TdvParams(ctypes.Structure):
                 ("ewald_accuracy", ctypes.c_double),
                 ("fp32_order", ctypes.c_int),
                 ("fp32_criterion", ctypes.c_int),
-                ("fp32_carry", ctypes.c_int)]
+                ("fp32_carry", ctypes.c_int),
+                ("fp32_waves", ctypes.c_int)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -127,7 +128,7 @@ class Engine:
                  softening=0.01, theta_bh=0.1, alpha_rel=0.02, vector_size=64,
                  compat=True, device=0, ewald_table=None, precision="fp64",
                  ewald_size=0.125, ewald_accuracy=0.1, fp32_order=2,
-                 fp32_criterion=2, fp32_carry=16):
+                 fp32_criterion=2, fp32_carry=16, fp32_waves=0):
         self.lib = load_library()
         prec = {"fp64": 0, "fp32": 1}[precision]
         self.params = TdvParams(int(periodic), float(boxsize), float(gravity),
@@ -136,7 +137,7 @@ class Engine:
                                 int(compat), int(device), prec,
                                 float(ewald_size), float(ewald_accuracy),
                                 int(fp32_order), int(fp32_criterion),
-                                int(fp32_carry))
+                                int(fp32_carry), int(fp32_waves))
         self._c = _ctx()
         rc = self.lib.tdv_create(ctypes.byref(self.params),
                                  ctypes.byref(self._c))
