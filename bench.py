@@ -45,6 +45,8 @@ WORKLOADS = {
                "128^3 (2,097,152 particles), Ewald-corrected"),
     "c2_small": dict(ngrid=64, seed=7, alpha=0.005, desc="periodic "
                      "Zel'dovich box 64^3, Ewald-corrected"),
+    "c3": dict(ngrid=256, seed=7, alpha=0.005, desc="periodic Zel'dovich box "
+               "256^3 (16,777,216 particles), Ewald-corrected"),
 }
 # the FP32 group walk's settings (the FP64 walk keeps the reference's)
 FP32 = dict(alpha=0.01, fp32_criterion=3, fp32_order=2, ewald_size=0.125,
@@ -166,12 +168,116 @@ def cpu_baseline(pos, m, soft, w, sample, nthreads, gpu_last, gpu_cost):
                       % (t1 - t0, t2 - t1, n, sample, t4 - t3, same_cost)}
 
 
+C4 = dict(n=1 << 23, nhalo=256, seed=11, desc="clustered periodic box: 256 "
+          "Hernquist halos (dN/dM ~ M^-1.9 over 3 decades, r_s ~ M^(1/3)) + "
+          "20% uniform background, 2^23 = 8,388,608 particles, 8 time bins")
+
+
+def bench_c4(args, f32):
+    """The clustered box with hierarchical time bins: one full step (every
+    particle active), then one substep per bin level with the active set of
+    bins <= b (Make_Active_Particle_List, timestep.c:280-299), the tree
+    kept.  Each substep runs the FP32 walk eagerly and as a replay of the
+    captured walk graph (safe_last_accel + walk records + the persistent
+    work-queue walk, one launch for any active set)."""
+    from tandav_amd import Engine, ics
+    n = args.c4_n or C4["n"]
+    log("generating clustered box, N = %d" % n)
+    pos, _, m = ics.clustered_box(n, nhalo=C4["nhalo"], seed=C4["seed"])
+    soft = 1.0 / (40.0 * n ** (1.0 / 3.0))
+    e = Engine(periodic=True, boxsize=1.0, softening=soft, theta_bh=0.1,
+               alpha_rel=f32["alpha"], compat=False, precision="fp32",
+               ewald_size=f32["ewald_size"],
+               ewald_accuracy=f32["ewald_accuracy"],
+               fp32_order=f32["fp32_order"],
+               fp32_criterion=f32["fp32_criterion"], fp32_queue=1)
+    e.set_particles(pos, m)
+    e.setup_domain()
+    e.domain_decomposition()
+    log("c4: prepare step")
+    e.compute_acceleration(prepare_step=True)
+    for i in range(args.warmup):
+        e.domain_decomposition()
+        e.compute_acceleration(prepare_step=False)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        e.domain_decomposition()
+        e.compute_acceleration(prepare_step=False)
+        log("c4: timed full step done")
+    t_step = (time.perf_counter() - t0) / args.steps
+    stage = e.stage_times()
+    newton, ewald = e.interaction_counts()
+    acc = e.get("acc")["acc"]
+    g = np.random.Generator(np.random.Philox(5))
+    xidx = np.sort(g.choice(n, min(n, args.exact_sample), replace=False))
+    t0 = time.perf_counter()
+    exact = e.forcetest_exact(xidx)
+    log("c4: exact-Ewald direct sum on %d particles %.1fs"
+        % (len(xidx), time.perf_counter() - t0))
+    err = errors(acc[xidx], exact)
+    # time bins from |a|: the fastest 0.05% in bin 0 ... everything by bin 7
+    amag = np.linalg.norm(acc, axis=1)
+    order = np.argsort(-amag, kind="stable")
+    frac = 0.0005 * (2000.0 ** (np.arange(8) / 7.0))
+    bins = np.empty(n, np.int32)
+    lo = 0
+    for b in range(8):
+        hi = n if b == 7 else int(round(frac[b] * n))
+        bins[order[lo:hi]] = b
+        lo = hi
+    # the graph needs one eager walk on this tree; capture, then substeps
+    e.compute_acceleration(prepare_step=False)
+    e.capture_walk()
+    subs = []
+    for b in range(8):
+        act = np.flatnonzero(bins <= b).astype(np.int32)
+        e.set_active(act)
+        t0 = time.perf_counter()
+        e.compute_acceleration(prepare_step=False)
+        t_eager = time.perf_counter() - t0
+        walk_eager = e.stage_times()["walk"]
+        e.set_active(act)
+        t0 = time.perf_counter()
+        e.launch_walk()
+        t_graph = time.perf_counter() - t0
+        walk_graph = e.stage_times()["walk"]
+        subs.append({"bin": b, "active_fraction": len(act) / n,
+                     "n_active": int(len(act)), "groups": (len(act) + 63) // 64,
+                     "eager_ms": t_eager * 1e3, "eager_walk_ms": walk_eager,
+                     "graph_ms": t_graph * 1e3, "graph_walk_ms": walk_graph})
+        log("c4: bins <= %d, %.4f%% active: eager %.2f ms, graph %.2f ms" % (
+            b, 100.0 * len(act) / n, t_eager * 1e3, t_graph * 1e3))
+    line = {
+        "metric": "tree-force evals/s per step (FP32 group walk), clustered "
+                  "box, and substep time vs active fraction",
+        "value": n / t_step, "unit": "force-evals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_step * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": C4["desc"] if n == C4["n"] else
+                   C4["desc"].replace("2^23 = 8,388,608", str(n)),
+                   "n_particles": n, "softening": soft,
+                   "fp32": dict(f32, fp32_queue=1),
+                   "error_sample_exact": len(xidx),
+                   "parallelism": "single GPU"},
+        "full_step": {"stages_ms": {k: round(v, 4) for k, v in stage.items()},
+                      "interactions_per_particle": (newton + ewald) / n,
+                      "err_vs_exact_ewald": err},
+        "substeps": subs,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c2",
+                    choices=sorted(WORKLOADS) + ["c4"])
+    ap.add_argument("--c4-n", type=int, default=0,
+                    help="particles of the clustered box (default 2^23)")
     ap.add_argument("--fp32-alpha", type=float, default=FP32["alpha"])
     ap.add_argument("--ewald-size", type=float, default=FP32["ewald_size"])
     ap.add_argument("--ewald-accuracy", type=float,
@@ -183,6 +289,10 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp32", "fp64"], default=None)
+    ap.add_argument("--alphas", default="",
+                    help="comma-separated extra opening criteria for the FP32 "
+                         "walk, each run and scored like the main one "
+                         "(the 256^3 box reports 0.0025,0.005,0.01)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (one plain copy "
                          "kernel, the reference point for the HBM byte "
@@ -204,6 +314,13 @@ def main():
               % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    f32 = dict(FP32, alpha=args.fp32_alpha, ewald_size=args.ewald_size,
+               ewald_accuracy=args.ewald_accuracy,
+               fp32_criterion=args.fp32_criterion)
+    if args.workload == "c4":
+        bench_c4(args, f32)
+        return
+
     log("generating %s" % args.workload)
     pos, vel, m, soft, w = make_workload(args.workload)
     n = len(m)
@@ -211,9 +328,6 @@ def main():
     g = np.random.Generator(np.random.Philox(5))
     idx = np.sort(g.choice(n, min(n, args.error_sample), replace=False))
 
-    f32 = dict(FP32, alpha=args.fp32_alpha, ewald_size=args.ewald_size,
-               ewald_accuracy=args.ewald_accuracy,
-               fp32_criterion=args.fp32_criterion)
     xidx = idx[np.sort(np.random.Generator(np.random.Philox(6)).choice(
         len(idx), min(len(idx), args.exact_sample), replace=False))]
     res = {}
@@ -243,6 +357,23 @@ def main():
                          newton=newton, ewald=ewald)
         log("%s: %.1f ms/step, err vs exact %s, vs table %s"
             % (prec, t_step * 1e3, res[prec]["err"], res[prec]["err_table"]))
+
+    scan = []
+    for a in [float(x) for x in args.alphas.split(",") if x]:
+        fa = dict(f32, alpha=a)
+        e, t_step, stage = run_gpu(pos, m, soft, w, args.steps, args.warmup,
+                                   "fp32", fa)
+        acc = e.get("acc")["acc"]
+        newton, ewald = e.interaction_counts()
+        scan.append({"alpha": a, "ms_per_step": t_step * 1e3,
+                     "force_evals_per_s": n / t_step,
+                     "walk_ms": stage["walk"],
+                     "interactions_per_particle": (newton + ewald) / n,
+                     "err_vs_exact_ewald": errors(acc[xidx], exact),
+                     "err_vs_table_ewald": errors(acc[idx], ref)})
+        log("fp32 alpha %g: %.1f ms/step, err vs exact %s" % (
+            a, t_step * 1e3, scan[-1]["err_vs_exact_ewald"]))
+        e.close()
 
     head = res["fp32"] if "fp32" in res else res["fp64"]
     if "fp32" in res:
@@ -317,6 +448,8 @@ def main():
         line["accuracy"][prec + "_meets_bar"] = r["err"]["p99"] <= 2e-3
     if "fp32" in res and "fp64" in res:
         line["fp32_over_fp64"] = res["fp64"]["t_step"] / res["fp32"]["t_step"]
+    if scan:
+        line["fp32_alpha_scan"] = scan
     print(json.dumps(line), flush=True)
 
 

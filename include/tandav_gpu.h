@@ -73,6 +73,8 @@ typedef struct {
 	                          (0 = 16)                                        */
 	int fp32_waves;        /* FP32 walk: register budget for 6 or 8 waves per
 	                          SIMD (0 = the compiler's own allocation)        */
+	int fp32_queue;        /* FP32 walk: 1 = persistent work-queue kernel
+	                          (fixed grid, groups taken from a counter)       */
 } tdv_params;
 
 enum { TDV_WALK_FP64 = 0, TDV_WALK_FP32 = 1 };
@@ -140,6 +142,18 @@ int tdv_tree_acceleration(tdv_context *ctx, int use_bh);
  * decomposition happened, BH walk + relative walk on the prepare step,
  * relative walk otherwise. */
 int tdv_compute_acceleration(tdv_context *ctx, int prepare_step);
+
+/* A regular step's FP32 walk (safe_last_accel, accel.c:28-43, the walk
+ * records and the work-queue walk over the current active list) captured as
+ * a HIP graph on the context's stream, for replay on substeps that reuse
+ * the tree (hierarchical time steps: Active_Particle_List changes,
+ * positions drift through tdv_update_particles, the tree stays).  Needs one
+ * eager FP32 walk on the current tree first; a sort, rebuild or new
+ * particles drop the graph.  Engine-side; no counterpart in the reference. */
+int tdv_capture_walk(tdv_context *ctx);
+/* Replay the captured walk (Compute_Acceleration on a regular step without
+ * a rebuild). */
+int tdv_launch_walk(tdv_context *ctx);
 
 /* Gravity_Forcetest (Gravity/forcetest.c:24-174): FP64 direct summation
  * over all particles (plus the tabulated Ewald correction of every pair on

@@ -86,8 +86,28 @@ static void dfree(T *&p)
 	p = nullptr;
 }
 
+/* a captured walk holds the buffers' addresses: drop it whenever a buffer
+ * may move (sort, rebuild, new particles, reallocation) */
+void tdv_graph_drop(tdv_context *c)
+{
+	if (c->graph_exec)
+		(void)hipGraphExecDestroy(c->graph_exec);
+	if (c->graph)
+		(void)hipGraphDestroy(c->graph);
+	c->graph_exec = nullptr;
+	c->graph = nullptr;
+}
+
+static int set_nactive(tdv_context *c, int nact)
+{
+	c->nactive = nact;
+	return tdv_check(c, hipMemcpyAsync(c->d_nactive, &c->nactive, sizeof(int),
+			hipMemcpyHostToDevice, c->stream), "nactive");
+}
+
 static void free_particles(tdv_context *c)
 {
+	tdv_graph_drop(c);
 	for (int d = 0; d < 3; d++) {
 		dfree(c->pos[d]);
 		dfree(c->vel[d]);
@@ -261,6 +281,8 @@ int tdv_create(const tdv_params *p, tdv_context **out)
 		return TDV_ERR_ARG;
 	if (p->fp32_waves != 0 && p->fp32_waves != 6 && p->fp32_waves != 8)
 		return TDV_ERR_ARG;
+	if (p->fp32_queue != 0 && p->fp32_queue != 1)
+		return TDV_ERR_ARG;
 	tdv_context *c = new (std::nothrow) tdv_context();
 	if (!c)
 		return TDV_ERR_NOMEM;
@@ -279,6 +301,10 @@ int tdv_create(const tdv_params *p, tdv_context **out)
 		r = dalloc(c, &c->d_err, 1);
 	if (!r)
 		r = dalloc(c, &c->d_count, 2);
+	if (!r)
+		r = dalloc(c, &c->d_nactive, 1);
+	if (!r)
+		r = dalloc(c, &c->d_queue, 1);
 	if (!r)
 		r = tdv_check(c, hipMemset(c->d_err, 0, sizeof(unsigned)), "memset");
 	if (!r)
@@ -316,8 +342,11 @@ void tdv_destroy(tdv_context *c)
 	dfree(c->q32);
 	dfree(c->tq32);
 	dfree(c->top32);
+	tdv_graph_drop(c);
 	dfree(c->d_err);
 	dfree(c->d_count);
+	dfree(c->d_nactive);
+	dfree(c->d_queue);
 	for (int e = 0; e < EV_N; e++)
 		if (c->ev[e])
 			(void)hipEventDestroy(c->ev[e]);
@@ -392,7 +421,8 @@ int tdv_set_particles(tdv_context *c, size_t n, const double *pos[3],
 		act[i] = (int)i;
 	TDV_HIP(c, hipMemcpyAsync(c->active, act.data(), n * sizeof(int),
 			hipMemcpyHostToDevice, c->stream));
-	c->nactive = (int)n;
+	TDV_TRY(set_nactive(c, (int)n));
+	tdv_graph_drop(c);
 	TDV_HIP(c, hipStreamSynchronize(c->stream));
 	c->have_domain = 0;
 	c->have_tree = 0;
@@ -506,8 +536,9 @@ static int keys_sort_reorder(tdv_context *c)
 	if ((size_t)c->nactive != n) {
 		hipLaunchKernelGGL(k_iota, dim3(tdv_blocks(n, 256)), dim3(256), 0,
 				c->stream, (int)n, c->active);
-		c->nactive = (int)n;
+		TDV_TRY(set_nactive(c, (int)n));
 	}
+	tdv_graph_drop(c);               /* columns were swapped */
 	(void)hipEventRecord(c->ev[EV_REORDER1], c->stream);
 	TDV_HIP(c, hipGetLastError());
 	TDV_TRY(tdv_check_err_flags(c));
@@ -588,6 +619,7 @@ int tdv_set_top_nodes(tdv_context *c, const void *records, int ntop)
 	TDV_TRY(up(c->top.pos, pos.data(), ntop * 12));
 	TDV_TRY(up(c->top.ntop, &ntop, 4));
 	c->ntop = ntop;
+	tdv_graph_drop(c);
 	TDV_TRY(tdv_assign_top(c));
 	TDV_HIP(c, hipStreamSynchronize(c->stream));
 	c->have_top = 1;
@@ -667,6 +699,52 @@ int tdv_compute_acceleration(tdv_context *c, int prepare_step)
 		tdv_launch_safe_last_accel(c);
 	}
 	return tdv_tree_acceleration(c, 0);
+}
+
+int tdv_capture_walk(tdv_context *c)
+{
+	if (!c || !c->have_tree || c->par.precision != TDV_WALK_FP32)
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_capture_walk: FP32 walk over a "
+				"built tree only");
+	if (!c->p32 || (c->par.periodic && !c->ewald16_ok)
+			|| c->n32_cap < c->node_cap || c->top32_cap < c->top_cap)
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_capture_walk: run one walk on "
+				"this tree first");
+	tdv_graph_drop(c);
+	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	TDV_HIP(c, hipStreamBeginCapture(c->stream,
+			hipStreamCaptureModeThreadLocal));
+	c->capturing = 1;
+	tdv_launch_safe_last_accel_dev(c);
+	int r = tdv_walk32(c, 0);
+	c->capturing = 0;
+	hipGraph_t g = nullptr;
+	const hipError_t e = hipStreamEndCapture(c->stream, &g);
+	if (r) {
+		if (g)
+			(void)hipGraphDestroy(g);
+		return r;
+	}
+	TDV_TRY(tdv_check(c, e, "hipStreamEndCapture"));
+	c->graph = g;
+	TDV_TRY(tdv_check(c, hipGraphInstantiate(&c->graph_exec, g, nullptr,
+			nullptr, 0), "hipGraphInstantiate"));
+	return 0;
+}
+
+int tdv_launch_walk(tdv_context *c)
+{
+	if (!c || !c->graph_exec)
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_launch_walk: nothing captured");
+	(void)hipEventRecord(c->ev[EV_WALK0], c->stream);
+	TDV_HIP(c, hipGraphLaunch(c->graph_exec, c->stream));
+	(void)hipEventRecord(c->ev[EV_WALK1], c->stream);
+	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	(void)hipEventElapsedTime(&c->stage_ms[5], c->ev[EV_WALK0], c->ev[EV_WALK1]);
+	c->stage_ms[6] = 0.f;
+	TDV_HIP(c, hipMemcpy(c->count, c->d_count, sizeof(c->count),
+			hipMemcpyDeviceToHost));
+	return tdv_check_err_flags(c);
 }
 
 static int forcetest(tdv_context *c, int nt, const int *idx, double *acc_out,
@@ -1037,7 +1115,7 @@ int tdv_set_active(tdv_context *c, int nactive, const int *list)
 	if (nactive)
 		TDV_HIP(c, hipMemcpyAsync(c->active, list, nactive * sizeof(int),
 				hipMemcpyHostToDevice, c->stream));
-	c->nactive = nactive;
+	TDV_TRY(set_nactive(c, nactive));
 	return tdv_check(c, hipStreamSynchronize(c->stream), "tdv_set_active");
 }
 

@@ -65,6 +65,12 @@ struct tdv_context {
 	double4 *pm;                     /* {x, y, z, m} for the walk */
 	int *active;                     /* active list */
 	int nactive;
+	int *d_nactive;                  /* nactive on the device (the queue
+	                                    walk and captured graphs read it) */
+	unsigned *d_queue;               /* work-queue counter of the walk */
+	hipGraph_t graph;                /* captured regular-step walk */
+	hipGraphExec_t graph_exec;
+	int capturing;
 
 	/* sort scratch */
 	uint64_t *sk[2];
@@ -144,6 +150,9 @@ int tdv_build_tree(tdv_context *c);
 int tdv_walk64(tdv_context *c, int use_bh);
 int tdv_walk64_ewald(tdv_context *c, int use_bh);
 int tdv_walk32(tdv_context *c, int use_bh);
+void tdv_launch_safe_last_accel_dev(tdv_context *c);
+void tdv_graph_drop(tdv_context *c);
+#define TDV_QUEUE_GRID 8192              /* 256 CUs x 4 SIMDs x 8 waves */
 int tdv_direct_sum(tdv_context *c, int nt, const int *idx_dev, double *out_dev);
 int tdv_direct_sum_exact(tdv_context *c, int nt, const int *idx_dev,
 		double *out_dev);

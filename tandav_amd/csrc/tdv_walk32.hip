@@ -471,7 +471,7 @@ __device__ __forceinline__ bool ewald_ok(float m, float s, float rx, float ry,
 			<= crit * w.ewald_acc;
 }
 
-__device__ __forceinline__ void walk32_body(int nact,
+__device__ __forceinline__ void walk32_body(int group, int nact,
 		const int *__restrict__ active, const int4 *__restrict__ p32,
 		const double *__restrict__ last_acc, TopNodes T,
 		const int4 *__restrict__ a32, const int4 *__restrict__ b32,
@@ -483,7 +483,7 @@ __device__ __forceinline__ void walk32_body(int nact,
 		const int4 *__restrict__ top32, const float4 *__restrict__ q32,
 		const float4 *__restrict__ tq32, const float4 *__restrict__ p32o)
 {
-	const int gid = blockIdx.x * 64 + threadIdx.x;
+	const int gid = group * 64 + (int)threadIdx.x;
 	const bool live = gid < nact;
 	const int ip = live ? active[gid] : active[nact - 1];
 	const int4 me = p32[ip];
@@ -684,30 +684,82 @@ __device__ __forceinline__ void walk32_body(int nact,
 		double *__restrict__ cost, unsigned long long *counter, \
 		const int4 *__restrict__ top32, const float4 *__restrict__ q32, \
 		const float4 *__restrict__ tq32, const float4 *__restrict__ p32o
-#define TDV_WALK32_CALL walk32_body(nact, active, p32, last_acc, T, a32, b32, \
-		nc, tab, w, ax, ay, az, gx, gy, gz, cost, counter, top32, q32, tq32, \
-		p32o)
+#define TDV_WALK32_CALL(g) walk32_body(g, nact, active, p32, last_acc, T, \
+		a32, b32, nc, tab, w, ax, ay, az, gx, gy, gz, cost, counter, top32, \
+		q32, tq32, p32o)
 
 __global__ void __launch_bounds__(64) k_walk32(TDV_WALK32_ARGS)
 {
-	TDV_WALK32_CALL;
+	TDV_WALK32_CALL(blockIdx.x);
 }
 
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
 k_walk32_w6(TDV_WALK32_ARGS)
 {
-	TDV_WALK32_CALL;
+	TDV_WALK32_CALL(blockIdx.x);
 }
 
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8)))
 k_walk32_w8(TDV_WALK32_ARGS)
 {
-	TDV_WALK32_CALL;
+	TDV_WALK32_CALL(blockIdx.x);
+}
+
+/* Persistent form: a fixed grid whose waves take 64-particle groups from a
+ * work queue (one atomic counter) until it is exhausted, with the active
+ * count read from device memory, so one launch serves any active subset and
+ * a captured graph can be replayed for every substep.  Every wave leaves
+ * once the counter has passed the last group. */
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8)))
+k_walk32_queue(const int *__restrict__ nact_dev, unsigned *__restrict__ queue,
+		TDV_WALK32_ARGS)
+{
+	nact = *nact_dev;
+	const int ngroups = (nact + 63) / 64;
+	for (;;) {
+		/* The ticket: every lane takes part in the atomic and only lane 0
+		 * adds, so lane 0's old value is the group; readfirstlane makes it
+		 * an SGPR and the exit a scalar branch.  No branch on the lane
+		 * here: the compiler merges a `threadIdx.x == 0` test at the loop
+		 * head with the one that ends the body, after which lanes 1-63
+		 * loop alone and read the ticket from an inactive lane 0 (zero),
+		 * i.e. walk group 0 forever. */
+		const unsigned t = atomicAdd(queue, threadIdx.x == 0 ? 1u : 0u);
+		const int g = __builtin_amdgcn_readfirstlane((int)t);
+		if (g >= ngroups)
+			break;
+		TDV_WALK32_CALL(g);
+	}
+}
+
+__global__ void k_walk32_queue_reset(unsigned *__restrict__ queue,
+		unsigned long long *__restrict__ counter)
+{
+	if (threadIdx.x == 0)
+		queue[0] = 0;
+	if (threadIdx.x < 2)
+		counter[threadIdx.x] = 0;
+}
+
+/* safe_last_accel (accel.c:28-43) with the active count on the device */
+__global__ void k_safe_last_accel_dev(const int *__restrict__ nact_dev,
+		const int *__restrict__ active, double *__restrict__ ax,
+		double *__restrict__ ay, double *__restrict__ az,
+		double *__restrict__ last)
+{
+	const int g = blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= *nact_dev)
+		return;
+	const int i = active[g];
+	last[i] = sqrt(ax[i] * ax[i] + ay[i] * ay[i] + az[i] * az[i]);
+	ax[i] = 0;
+	ay[i] = 0;
+	az[i] = 0;
 }
 
 int tdv_walk32(tdv_context *c, int use_bh)
 {
-	if (!c->nactive)
+	if (!c->nactive && !c->capturing)
 		return 0;
 	const size_t n = c->n, nn = c->nnodes;
 	/* fixed-point frame: 2^32 per periodic box, 2^31 per domain otherwise */
@@ -722,6 +774,7 @@ int tdv_walk32(tdv_context *c, int use_bh)
 		TDV_HIP(c, hipMalloc((void **)&c->p32o, c->cap * sizeof(float4)));
 	}
 	if (c->n32_cap < c->node_cap) {
+		tdv_graph_drop(c);               /* the records move */
 		if (c->a32)
 			(void)hipFree(c->a32);
 		if (c->b32)
@@ -741,6 +794,7 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	hipLaunchKernelGGL(k_prep32_particles, dim3(tdv_blocks(n, 256)), dim3(256),
 			0, c->stream, n, c->pm, o[0], o[1], o[2], S, mscale, c->p32);
 	if (c->top32_cap < c->top_cap) {
+		tdv_graph_drop(c);
 		if (c->top32)
 			(void)hipFree(c->top32);
 		if (c->tq32)
@@ -826,6 +880,18 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	w.carry_every = c->par.fp32_carry > 0 ? c->par.fp32_carry : 16;
 	w.quad = quad ? 1 : 0;
 	w.crit3 = c->par.fp32_criterion == 3;
+	if (c->par.fp32_queue || c->capturing) {
+		/* grid: enough waves to fill every SIMD at the 8-wave budget */
+		hipLaunchKernelGGL(k_walk32_queue_reset, dim3(1), dim3(64), 0,
+				c->stream, c->d_queue, c->d_count);
+		hipLaunchKernelGGL(k_walk32_queue, dim3(TDV_QUEUE_GRID), dim3(64), 0,
+				c->stream, c->d_nactive, c->d_queue, 0, c->active, c->p32,
+				c->last_acc, c->top, c->a32, c->b32, c->node_c, c->ewald16, w,
+				c->acc[0], c->acc[1], c->acc[2], c->gacc[0], c->gacc[1],
+				c->gacc[2], c->cost, c->d_count, c->top32, c->q32, c->tq32,
+				c->p32o);
+		return tdv_check(c, hipGetLastError(), "k_walk32_queue");
+	}
 	auto kern = c->par.fp32_waves == 6 ? k_walk32_w6
 			: c->par.fp32_waves == 8 ? k_walk32_w8 : k_walk32;
 	hipLaunchKernelGGL(kern, dim3(tdv_blocks((size_t)c->nactive, 64)),
@@ -834,4 +900,11 @@ int tdv_walk32(tdv_context *c, int use_bh)
 			c->acc[1], c->acc[2], c->gacc[0], c->gacc[1], c->gacc[2], c->cost,
 			c->d_count, c->top32, c->q32, c->tq32, c->p32o);
 	return tdv_check(c, hipGetLastError(), "k_walk32");
+}
+
+void tdv_launch_safe_last_accel_dev(tdv_context *c)
+{
+	hipLaunchKernelGGL(k_safe_last_accel_dev, dim3(tdv_blocks(c->n, 256)),
+			dim3(256), 0, c->stream, c->d_nactive, c->active, c->acc[0],
+			c->acc[1], c->acc[2], c->last_acc);
 }

@@ -39,7 +39,8 @@ class TdvParams(ctypes.Structure):
                 ("fp32_order", ctypes.c_int),
                 ("fp32_criterion", ctypes.c_int),
                 ("fp32_carry", ctypes.c_int),
-                ("fp32_waves", ctypes.c_int)]
+                ("fp32_waves", ctypes.c_int),
+                ("fp32_queue", ctypes.c_int)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -104,6 +105,8 @@ def load_library(path=LIB_PATH):
     L.tdv_set_active.argtypes = [_ctx, ctypes.c_int, _vp]
     L.tdv_set_tree_moments.argtypes = [_ctx, _vp, _sz, _vp, ctypes.c_int]
     L.tdv_get_ewald_derivs.argtypes = [_ctx, _dp]
+    L.tdv_capture_walk.argtypes = [_ctx]
+    L.tdv_launch_walk.argtypes = [_ctx]
     L.tdv_num_particles.argtypes = [_ctx]
     L.tdv_num_particles.restype = _sz
     _lib = L
@@ -128,7 +131,8 @@ class Engine:
                  softening=0.01, theta_bh=0.1, alpha_rel=0.02, vector_size=64,
                  compat=True, device=0, ewald_table=None, precision="fp64",
                  ewald_size=0.125, ewald_accuracy=0.1, fp32_order=2,
-                 fp32_criterion=2, fp32_carry=16, fp32_waves=0):
+                 fp32_criterion=2, fp32_carry=16, fp32_waves=0,
+                 fp32_queue=0):
         self.lib = load_library()
         prec = {"fp64": 0, "fp32": 1}[precision]
         self.params = TdvParams(int(periodic), float(boxsize), float(gravity),
@@ -137,7 +141,8 @@ class Engine:
                                 int(compat), int(device), prec,
                                 float(ewald_size), float(ewald_accuracy),
                                 int(fp32_order), int(fp32_criterion),
-                                int(fp32_carry), int(fp32_waves))
+                                int(fp32_carry), int(fp32_waves),
+                                int(fp32_queue))
         self._c = _ctx()
         rc = self.lib.tdv_create(ctypes.byref(self.params),
                                  ctypes.byref(self._c))
@@ -276,6 +281,14 @@ class Engine:
         """Compute_Acceleration (accel.c:15-71)."""
         self._ok(self.lib.tdv_compute_acceleration(self._c, int(prepare_step)),
                  "tdv_compute_acceleration")
+
+    def capture_walk(self):
+        """Capture the regular step's FP32 walk as a HIP graph."""
+        self._ok(self.lib.tdv_capture_walk(self._c), "tdv_capture_walk")
+
+    def launch_walk(self):
+        """Replay the captured walk."""
+        self._ok(self.lib.tdv_launch_walk(self._c), "tdv_launch_walk")
 
     def forcetest_exact(self, idx):
         """forcetest with every pair's Ewald correction summed exactly
