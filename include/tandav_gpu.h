@@ -75,6 +75,10 @@ typedef struct {
 	                          SIMD (0 = the compiler's own allocation)        */
 	int fp32_queue;        /* FP32 walk: 1 = persistent work-queue kernel
 	                          (fixed grid, groups taken from a counter)       */
+	int fp32_split;        /* FP32 walk with few active groups: parts each
+	                          group's subtree walks are cut into, one wave per
+	                          part (summed in part order); 0 = up to 64 parts,
+	                          1 = never split, n >= 2 = up to n (<= 256)      */
 } tdv_params;
 
 enum { TDV_WALK_FP64 = 0, TDV_WALK_FP32 = 1 };
@@ -197,6 +201,17 @@ int tdv_set_ewald_table(tdv_context *ctx, const double *table4);
  * walk uses them to take a node's correction at its centre of mass to
  * second order; the reference has no counterpart. */
 int tdv_get_ewald_derivs(tdv_context *ctx, double *s10);
+
+/* The FP32 walk's quadrupole records of the last FP32 walk: per tree node
+ * (tdv_get_tree's order) and per top node 8 floats, {Txx, Tyy, Txy, Txz,
+ * Tyz, 0, 0, 0}, T = sum m (3 y y^T - |y|^2 I) about the node's centre of
+ * mass (a top node's float one), masses G m S^2 and lengths in the walk's
+ * fixed units (S = 2^32 per box on periodic runs, 2^31 per domain
+ * otherwise); node_first gets each node's first particle (-1 for
+ * terminators).  Any output may be NULL.  A diagnostic of the FP32 walk; the
+ * reference has no counterpart. */
+int tdv_get_walk32_quadrupoles(tdv_context *ctx, float *nodes, float *top,
+		int *node_first);
 
 /* Domain (domain.h:58-63) after the last decomposition. */
 int tdv_get_domain(tdv_context *ctx, double *size, double origin[3],

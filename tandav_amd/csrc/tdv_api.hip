@@ -283,6 +283,8 @@ int tdv_create(const tdv_params *p, tdv_context **out)
 		return TDV_ERR_ARG;
 	if (p->fp32_queue != 0 && p->fp32_queue != 1)
 		return TDV_ERR_ARG;
+	if (p->fp32_split < 0 || p->fp32_split > 256)
+		return TDV_ERR_ARG;
 	tdv_context *c = new (std::nothrow) tdv_context();
 	if (!c)
 		return TDV_ERR_NOMEM;
@@ -340,6 +342,8 @@ void tdv_destroy(tdv_context *c)
 	dfree(c->a32);
 	dfree(c->b32);
 	dfree(c->q32);
+	dfree(c->q_s6);
+	dfree(c->q_arrived);
 	dfree(c->tq32);
 	dfree(c->top32);
 	tdv_graph_drop(c);
@@ -347,6 +351,7 @@ void tdv_destroy(tdv_context *c)
 	dfree(c->d_count);
 	dfree(c->d_nactive);
 	dfree(c->d_queue);
+	dfree(c->partial);
 	for (int e = 0; e < EV_N; e++)
 		if (c->ev[e])
 			(void)hipEventDestroy(c->ev[e]);
@@ -923,6 +928,30 @@ int tdv_get_ewald_derivs(tdv_context *c, double *s10)
 	TDV_HIP(c, hipMemcpyAsync(s10, c->ewald_s, 10 * (size_t)TDV_NE3 * 8,
 			hipMemcpyDeviceToHost, c->stream));
 	return tdv_check(c, hipStreamSynchronize(c->stream), "ewald derivs");
+}
+
+int tdv_get_walk32_quadrupoles(tdv_context *c, float *nodes, float *top,
+		int *node_first)
+{
+	if (!c || !c->have_tree || !c->q32 || !c->tq32
+			|| c->n32_cap < c->nnodes || c->top32_cap < c->ntop)
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_get_walk32_quadrupoles: no "
+				"FP32 walk with quadrupoles on this tree");
+	if (nodes && c->nnodes)
+		TDV_HIP(c, hipMemcpyAsync(nodes, c->q32, c->nnodes * 32,
+				hipMemcpyDeviceToHost, c->stream));
+	if (top && c->ntop)
+		TDV_HIP(c, hipMemcpyAsync(top, c->tq32, (size_t)c->ntop * 32,
+				hipMemcpyDeviceToHost, c->stream));
+	if (node_first && c->nnodes) {
+		std::vector<int4> nc(c->nnodes);
+		TDV_HIP(c, hipMemcpyAsync(nc.data(), c->node_c, c->nnodes * 16,
+				hipMemcpyDeviceToHost, c->stream));
+		TDV_HIP(c, hipStreamSynchronize(c->stream));
+		for (size_t j = 0; j < c->nnodes; j++)
+			node_first[j] = nc[j].w < 0 ? -1 : nc[j].w;
+	}
+	return tdv_check(c, hipStreamSynchronize(c->stream), "quadrupoles");
 }
 
 int tdv_get_domain(tdv_context *c, double *size, double origin[3],

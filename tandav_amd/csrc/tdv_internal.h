@@ -68,6 +68,9 @@ struct tdv_context {
 	int *d_nactive;                  /* nactive on the device (the queue
 	                                    walk and captured graphs read it) */
 	unsigned *d_queue;               /* work-queue counter of the walk */
+	double4 *partial;                /* split walk: TDV_SPLIT_ITEMS x 64 */
+	double *q_s6;                    /* nodes' FP64 second moments, 6 each */
+	unsigned *q_arrived;             /* children counted at each node */
 	hipGraph_t graph;                /* captured regular-step walk */
 	hipGraphExec_t graph_exec;
 	int capturing;
@@ -152,7 +155,12 @@ int tdv_walk64_ewald(tdv_context *c, int use_bh);
 int tdv_walk32(tdv_context *c, int use_bh);
 void tdv_launch_safe_last_accel_dev(tdv_context *c);
 void tdv_graph_drop(tdv_context *c);
-#define TDV_QUEUE_GRID 8192              /* 256 CUs x 4 SIMDs x 8 waves */
+#define TDV_QUEUE_GRID 8192              /* 256 CUs x 4 SIMDs x 8 waves;
+                                            waves beyond those resident
+                                            start later and find the queue
+                                            drained */
+#define TDV_SPLIT_ITEMS 16384            /* work items a split walk aims at */
+#define TDV_SPLIT_AUTO_MAX 64            /* parts per group, split 0 */
 int tdv_direct_sum(tdv_context *c, int nt, const int *idx_dev, double *out_dev);
 int tdv_direct_sum_exact(tdv_context *c, int nt, const int *idx_dev,
 		double *out_dev);

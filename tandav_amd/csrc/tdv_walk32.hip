@@ -167,73 +167,155 @@ __global__ void k_prep32_offsets(size_t n, const int4 *__restrict__ p32,
 	p32o[i] = o;
 }
 
-/* Traceless second moment T = sum m (3 y y^T - |y|^2 I), y = x - CoM, of
- * the particles [first, first + npart) of every node and bundle (and of
- * every top node), in FP64 about the FP64 centre of mass, one wavefront per node,
- * stored scaled (G S^2 masses, fixed lengths) as {Txx, Tyy, Txy, Txz},
- * {Tyz, 0, 0, 0}; Tzz = -Txx - Tyy. */
-__device__ __forceinline__ void quad_moments(const double4 *__restrict__ pm,
-		int first, int npart, double cx, double cy, double cz, double scale,
-		float4 *__restrict__ out, int lane)
+/* Quadrupoles of every node, bundle and top node, for the walk: the
+ * traceless moment T = 3 S - tr(S) I of the second moment S = sum m y y^T,
+ * y = x - CoM, in FP64 about the node's FP64 centre of mass (a top node's
+ * about its float centre of mass, the one the walk uses), stored scaled (G
+ * S^2 masses, fixed lengths) as {Txx, Tyy, Txy, Txz}, {Tyz, 0, 0, 0}; Tzz =
+ * -Txx - Tyy.
+ *
+ * Bottom-up in O(nodes + particles): every node covers the contiguous
+ * particle range [first, first + Npart) and its children (the nodes from
+ * j + 1 below j + DNext, siblings linked by DNext, bundles by + 1) cover
+ * sub-ranges of it in PH order.  A node without child nodes takes S from
+ * its particles; every node then counts itself at its parent (DUp), and the
+ * child that arrives last, which sees every sibling's S, forms the parent's
+ * S = sum_c (S_c + m_c d_c d_c^T), d_c = CoM_c - CoM, plus the particles no
+ * child covers, and climbs on.  Nobody waits: the others leave. */
+__device__ __forceinline__ void moments_add_particles(
+		const double4 *__restrict__ pm, int first, int count, double cx,
+		double cy, double cz, double *m6)
 {
-	double sxx = 0, syy = 0, szz = 0, sxy = 0, sxz = 0, syz = 0;
-	for (int p = first + lane; p < first + npart; p += 64) {
+	for (int p = first; p < first + count; p++) {
 		const double4 q = pm[p];
 		const double dx = q.x - cx, dy = q.y - cy, dz = q.z - cz;
 		const double mx = q.w * dx, my = q.w * dy;
-		sxx = fma(mx, dx, sxx);
-		syy = fma(my, dy, syy);
-		szz = fma(q.w * dz, dz, szz);
-		sxy = fma(mx, dy, sxy);
-		sxz = fma(mx, dz, sxz);
-		syz = fma(my, dz, syz);
-	}
-	for (int d = 32; d >= 1; d >>= 1) {
-		sxx += __shfl_xor(sxx, d, 64);
-		syy += __shfl_xor(syy, d, 64);
-		szz += __shfl_xor(szz, d, 64);
-		sxy += __shfl_xor(sxy, d, 64);
-		sxz += __shfl_xor(sxz, d, 64);
-		syz += __shfl_xor(syz, d, 64);
-	}
-	if (lane == 0) {
-		const double tr = sxx + syy + szz;
-		out[0] = make_float4((float)(scale * (3 * sxx - tr)),
-				(float)(scale * (3 * syy - tr)), (float)(scale * 3 * sxy),
-				(float)(scale * 3 * sxz));
-		out[1] = make_float4((float)(scale * 3 * syz), 0.f, 0.f, 0.f);
+		m6[0] = fma(mx, dx, m6[0]);
+		m6[1] = fma(my, dy, m6[1]);
+		m6[2] = fma(q.w * dz, dz, m6[2]);
+		m6[3] = fma(mx, dy, m6[3]);
+		m6[4] = fma(mx, dz, m6[4]);
+		m6[5] = fma(my, dz, m6[5]);
 	}
 }
 
-__global__ void __launch_bounds__(256) k_prep32_quad(int nn,
-		const int4 *__restrict__ nc, const double4 *__restrict__ na,
-		const double4 *__restrict__ pm, double scale, float4 *__restrict__ q)
+__device__ __forceinline__ void moments_store(const double *m6, double scale,
+		float4 *__restrict__ out)
 {
-	const int lane = threadIdx.x & 63;
-	const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
-	if (j >= nn)
-		return;
-	const int4 c = nc[j];
-	if (c.x == 0) {                  /* terminators: never used */
-		if (lane < 2)
-			q[2 * j + lane] = make_float4(0.f, 0.f, 0.f, 0.f);
-		return;
-	}
-	const double4 a = na[j];
-	quad_moments(pm, c.w, c.y, a.x, a.y, a.z, scale, q + 2 * j, lane);
+	const double tr = m6[0] + m6[1] + m6[2];
+	out[0] = make_float4((float)(scale * (3 * m6[0] - tr)),
+			(float)(scale * (3 * m6[1] - tr)), (float)(scale * 3 * m6[3]),
+			(float)(scale * 3 * m6[4]));
+	out[1] = make_float4((float)(scale * 3 * m6[5]), 0.f, 0.f, 0.f);
 }
 
-__global__ void __launch_bounds__(256) k_prep32_top_quad(int ntop,
-		const TopNodes T, const double4 *__restrict__ pm, double scale,
+/* the next sibling of child ch (DNext for nodes, + 1 for bundles) */
+__device__ __forceinline__ int next_sibling(const int4 *__restrict__ nc,
+		int ch)
+{
+	const int d = nc[ch].x;
+	return d > 0 ? ch + d : ch + 1;
+}
+
+__global__ void k_moments_clear(int nn, unsigned *__restrict__ arrived)
+{
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j < nn)
+		arrived[j] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_moments_up(int nn,
+		const int4 *__restrict__ nc, const int *__restrict__ up,
+		const double4 *__restrict__ na, const double4 *__restrict__ pm,
+		double *__restrict__ s6, unsigned *__restrict__ arrived, double scale,
 		float4 *__restrict__ q)
 {
-	const int lane = threadIdx.x & 63;
-	const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+	int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= nn)
+		return;
+	int4 c = nc[j];
+	if (c.w < 0) {                       /* terminators: never used */
+		q[2 * j] = make_float4(0.f, 0.f, 0.f, 0.f);
+		q[2 * j + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+		return;
+	}
+	if (c.x > 1)                         /* has child nodes: done by them */
+		return;
+	double m6[6] = { 0, 0, 0, 0, 0, 0 };
+	double4 a = na[j];
+	moments_add_particles(pm, c.w, c.y, a.x, a.y, a.z, m6);
+	for (;;) {
+		for (int k = 0; k < 6; k++)
+			s6[6 * (size_t)j + k] = m6[k];
+		moments_store(m6, scale, q + 2 * j);
+		if ((c.z >> TDV_BIT_TOP) & 1)
+			return;                      /* a subtree's root */
+		const int par = j - up[j];
+		const int4 pc = nc[par];
+		const int end = par + pc.x;
+		int nchild = 0;
+		for (int ch = par + 1; ch < end; ch = next_sibling(nc, ch))
+			nchild++;
+		__threadfence();                 /* this node's S before the count */
+		if (atomicAdd(&arrived[par], 1u) != (unsigned)(nchild - 1))
+			return;
+		__threadfence();                 /* every sibling's S after it */
+		j = par;
+		c = pc;
+		a = na[j];
+		for (int k = 0; k < 6; k++)
+			m6[k] = 0;
+		int covered = c.w;
+		for (int ch = j + 1; ch < end; ch = next_sibling(nc, ch)) {
+			const int4 cc = nc[ch];
+			moments_add_particles(pm, covered, cc.w - covered, a.x, a.y, a.z,
+					m6);
+			const double4 ca = na[ch];
+			const double dx = ca.x - a.x, dy = ca.y - a.y, dz = ca.z - a.z;
+			const double *sc = s6 + 6 * (size_t)ch;
+			const double mx = ca.w * dx, my = ca.w * dy;
+			m6[0] += sc[0] + mx * dx;
+			m6[1] += sc[1] + my * dy;
+			m6[2] += sc[2] + ca.w * dz * dz;
+			m6[3] += sc[3] + mx * dy;
+			m6[4] += sc[4] + mx * dz;
+			m6[5] += sc[5] + my * dz;
+			covered = cc.w + cc.y;
+		}
+		moments_add_particles(pm, covered, c.w + c.y - covered, a.x, a.y,
+				a.z, m6);
+	}
+}
+
+/* top nodes: from the subtree's root, shifted to the float centre of mass
+ * (S about a point P = S_cm + M (c - P)(c - P)^T), or from the particles of
+ * a top node without subtree */
+__global__ void k_moments_top(int ntop, const TopNodes T, int vs,
+		const double4 *__restrict__ na, const double4 *__restrict__ pm,
+		const double *__restrict__ s6, double scale, float4 *__restrict__ q)
+{
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= ntop)
 		return;
-	quad_moments(pm, T.first[t], T.npart[t], (double)T.com[3 * t],
-			(double)T.com[3 * t + 1], (double)T.com[3 * t + 2], scale,
-			q + 2 * t, lane);
+	const double cx = T.com[3 * t], cy = T.com[3 * t + 1],
+			cz = T.com[3 * t + 2];
+	double m6[6] = { 0, 0, 0, 0, 0, 0 };
+	if (T.npart[t] > vs) {
+		const int r = T.target[t];
+		const double4 a = na[r];
+		const double dx = a.x - cx, dy = a.y - cy, dz = a.z - cz;
+		const double mx = a.w * dx, my = a.w * dy;
+		const double *sr = s6 + 6 * (size_t)r;
+		m6[0] = sr[0] + mx * dx;
+		m6[1] = sr[1] + my * dy;
+		m6[2] = sr[2] + a.w * dz * dz;
+		m6[3] = sr[3] + mx * dy;
+		m6[4] = sr[4] + mx * dz;
+		m6[5] = sr[5] + my * dz;
+	} else {
+		moments_add_particles(pm, T.first[t], T.npart[t], cx, cy, cz, m6);
+	}
+	moments_store(m6, scale, q + 2 * t);
 }
 
 /* ---- the walk ---------------------------------------------------------- */
@@ -471,7 +553,31 @@ __device__ __forceinline__ bool ewald_ok(float m, float s, float rx, float ry,
 			<= crit * w.ewald_acc;
 }
 
-__device__ __forceinline__ void walk32_body(int group, int nact,
+/* The walk's decision at an internal node (tree_accel.c:249-262): the BH
+ * criterion, or the relative one or the particle within 0.6 s of the
+ * node's centre; (rx, ry, rz) to the CoM, (sx, sy, sz) to the centre. */
+__device__ __forceinline__ bool node_opens(float m, float s, float r2,
+		float sx, float sy, float sz, float crit, const Walk32Params &w)
+{
+	if (w.use_bh)
+		return s * s > r2 * w.theta_bh;
+	return rel_open(m, s, r2, crit, w) || (fabsf(sx) < 0.6f * s
+			&& fabsf(sy) < 0.6f * s && fabsf(sz) < 0.6f * s);
+}
+
+/* One wave walks `group` (64 active particles, one per lane) over part
+ * `part` of `nparts`: every subtree the group walks is cut into nparts
+ * contiguous node ranges and this wave takes range `part` of each, so
+ * several waves share a group whose walk is long (few active groups).  The
+ * walk visits nodes in increasing index order, so the walk of a range
+ * starts at the first node >= its start that the whole walk would visit:
+ * the descent below replays the whole walk's decisions on the range
+ * start's ancestors (and their Ewald state) without interacting.  Part 0
+ * also takes the top-node interactions.  With nparts == 1 the results go
+ * to the particles; otherwise to partial[part * nslot + slot], summed in
+ * part order by k_walk32_parts_sum. */
+__device__ __forceinline__ void walk32_body(int group, int part, int nparts,
+		double4 *__restrict__ partial, int nslot, int nact,
 		const int *__restrict__ active, const int4 *__restrict__ p32,
 		const double *__restrict__ last_acc, TopNodes T,
 		const int4 *__restrict__ a32, const int4 *__restrict__ b32,
@@ -505,6 +611,8 @@ __device__ __forceinline__ void walk32_body(int group, int nact,
 		/* the top node holds the particle (or is close): walk it */
 		want = want || r2 < 3.f * ts * ts;
 		if (!__any(want)) {
+			if (part != 0)
+				continue;
 			newton32(a, tm, rx, ry, rz, w);
 			if (w.quad) {
 				const float4 q0 = tq32[2 * j];
@@ -519,6 +627,8 @@ __device__ __forceinline__ void walk32_body(int group, int nact,
 			continue;
 		}
 		if (T.npart[j] <= w.vs) {
+			if (part != 0)
+				continue;
 			const int f = T.first[j], l = f + T.npart[j];
 			for (int q = f; q < l; q++) {
 				const int4 pq = p32[q];
@@ -533,9 +643,53 @@ __device__ __forceinline__ void walk32_body(int group, int nact,
 			continue;
 		}
 		const int start = T.target[j];
+		int lo = start, hi = T.term[j];
+		if (nparts > 1) {
+			const long long len = hi - start;
+			lo = start + (int)(len * part / nparts);
+			hi = start + (int)(len * (part + 1) / nparts);
+			if (lo >= hi)
+				continue;
+		}
 		int node = start;
 		int ew_until = w.periodic ? -1 : 0x7FFFFFFF;  /* correction done below */
+		while (node < lo) {
+			/* internal nodes have DNext > 0, the root's reaching the
+			 * terminator (tree_build.c:466-491) */
+			const int4 c = nc[node];
+			if (c.x < 0) {
+				node++;
+				continue;
+			}
+			if (c.x == 0) {                  /* only the terminator */
+				node = hi;
+				break;
+			}
+			const int skip = node + c.x;
+			if (skip <= lo) {            /* left before lo, open or not */
+				node = skip;
+				continue;
+			}
+			const int4 A = a32[node];
+			const int4 B = b32[node];
+			const float m = __int_as_float(A.w), s = __int_as_float(B.w);
+			const float rx = fdiff(A.x, me.x), ry = fdiff(A.y, me.y),
+					rz = fdiff(A.z, me.z);
+			const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+			const float sx = fdiff(B.x, me.x), sy = fdiff(B.y, me.y),
+					sz = fdiff(B.z, me.z);
+			if (!__any(node_opens(m, s, r2, sx, sy, sz, crit, w))) {
+				node = skip;             /* taken whole by an earlier part */
+				break;
+			}
+			if (node >= ew_until && __all(ewald_ok(m, s, rx, ry, rz, sx, sy,
+					sz, crit, false, w)))
+				ew_until = skip;
+			node++;
+		}
 		for (;;) {
+			if (node >= hi)
+				break;
 			const int4 c = nc[node];
 			if (c.x == 0 && node != start)
 				break;
@@ -612,17 +766,9 @@ __device__ __forceinline__ void walk32_body(int group, int nact,
 			const float rx = fdiff(A.x, me.x), ry = fdiff(A.y, me.y),
 					rz = fdiff(A.z, me.z);
 			const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
-			bool open;
 			const float sx = fdiff(B.x, me.x), sy = fdiff(B.y, me.y),
 					sz = fdiff(B.z, me.z);
-			if (w.use_bh) {
-				open = s * s > r2 * w.theta_bh;
-			} else {
-				open = rel_open(m, s, r2, crit, w);
-				open = open || (fabsf(sx) < 0.6f * s && fabsf(sy) < 0.6f * s
-						&& fabsf(sz) < 0.6f * s);
-			}
-			if (!__any(open)) {
+			if (!__any(node_opens(m, s, r2, sx, sy, sz, crit, w))) {
 				newton32(a, m, rx, ry, rz, w);
 				if (w.quad) {
 					const float4 q0 = q32[2 * node];
@@ -650,7 +796,11 @@ __device__ __forceinline__ void walk32_body(int group, int nact,
 		}
 	}
 	carry(a);
-	if (live) {
+	if (nparts > 1) {
+		if (live)
+			partial[(size_t)part * nslot + gid] = make_double4(a.X, a.Y, a.Z,
+					(double)(a.cost + a.ewald));
+	} else if (live) {
 		ax[ip] = ax[ip] + a.X;
 		ay[ip] = ay[ip] + a.Y;
 		az[ip] = az[ip] + a.Z;
@@ -684,9 +834,11 @@ __device__ __forceinline__ void walk32_body(int group, int nact,
 		double *__restrict__ cost, unsigned long long *counter, \
 		const int4 *__restrict__ top32, const float4 *__restrict__ q32, \
 		const float4 *__restrict__ tq32, const float4 *__restrict__ p32o
-#define TDV_WALK32_CALL(g) walk32_body(g, nact, active, p32, last_acc, T, \
-		a32, b32, nc, tab, w, ax, ay, az, gx, gy, gz, cost, counter, top32, \
-		q32, tq32, p32o)
+#define TDV_WALK32_CALL_PART(g, part, nparts, partial, nslot) \
+		walk32_body(g, part, nparts, partial, nslot, nact, active, p32, \
+		last_acc, T, a32, b32, nc, tab, w, ax, ay, az, gx, gy, gz, cost, \
+		counter, top32, q32, tq32, p32o)
+#define TDV_WALK32_CALL(g) TDV_WALK32_CALL_PART(g, 0, 1, nullptr, 0)
 
 __global__ void __launch_bounds__(64) k_walk32(TDV_WALK32_ARGS)
 {
@@ -705,17 +857,32 @@ k_walk32_w8(TDV_WALK32_ARGS)
 	TDV_WALK32_CALL(blockIdx.x);
 }
 
-/* Persistent form: a fixed grid whose waves take 64-particle groups from a
- * work queue (one atomic counter) until it is exhausted, with the active
- * count read from device memory, so one launch serves any active subset and
- * a captured graph can be replayed for every substep.  Every wave leaves
- * once the counter has passed the last group. */
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8)))
-k_walk32_queue(const int *__restrict__ nact_dev, unsigned *__restrict__ queue,
-		TDV_WALK32_ARGS)
+/* Parts per group: 1 when the active groups alone fill the chip, else
+ * enough that groups x parts reaches TDV_SPLIT_ITEMS work items, at most
+ * TDV_SPLIT_AUTO_MAX (split 0) or `split` (>= 2); split 1 never splits. */
+__host__ __device__ __forceinline__ int walk32_parts(int ngroups, int split)
+{
+	if (split == 1 || ngroups <= 0)
+		return 1;
+	const int kmax = split >= 2 ? split : TDV_SPLIT_AUTO_MAX;
+	const int k = TDV_SPLIT_ITEMS / ngroups;
+	return k < 1 ? 1 : k < kmax ? k : kmax;
+}
+
+/* Persistent form: a fixed grid whose waves take work items (a 64-particle
+ * group, or one part of one, see walk32_body) from a work queue (one atomic
+ * counter) until it is exhausted, with the active count read from device
+ * memory, so one launch serves any active subset and a captured graph can
+ * be replayed for every substep.  Items of one group are consecutive, so
+ * its parts run side by side.  Every wave leaves once the counter has
+ * passed the last item. */
+__global__ void __launch_bounds__(64) k_walk32_queue(const int *__restrict__ nact_dev, unsigned *__restrict__ queue,
+		int split, double4 *__restrict__ partial, TDV_WALK32_ARGS)
 {
 	nact = *nact_dev;
 	const int ngroups = (nact + 63) / 64;
+	const int nparts = walk32_parts(ngroups, split);
+	const int nitems = ngroups * nparts, nslot = ngroups * 64;
 	for (;;) {
 		/* The ticket: every lane takes part in the atomic and only lane 0
 		 * adds, so lane 0's old value is the group; readfirstlane makes it
@@ -725,11 +892,46 @@ k_walk32_queue(const int *__restrict__ nact_dev, unsigned *__restrict__ queue,
 		 * loop alone and read the ticket from an inactive lane 0 (zero),
 		 * i.e. walk group 0 forever. */
 		const unsigned t = atomicAdd(queue, threadIdx.x == 0 ? 1u : 0u);
-		const int g = __builtin_amdgcn_readfirstlane((int)t);
-		if (g >= ngroups)
+		const int item = __builtin_amdgcn_readfirstlane((int)t);
+		if (item >= nitems)
 			break;
-		TDV_WALK32_CALL(g);
+		const int g = item / nparts;
+		TDV_WALK32_CALL_PART(g, item - g * nparts, nparts, partial, nslot);
 	}
+}
+
+/* The parts' sums of a split walk, in part order, to the particles (as
+ * walk32_body writes them when it does not split) */
+__global__ void k_walk32_parts_sum(const int *__restrict__ nact_dev, int split,
+		const int *__restrict__ active, const double4 *__restrict__ partial,
+		double *__restrict__ ax, double *__restrict__ ay,
+		double *__restrict__ az, double *__restrict__ gx,
+		double *__restrict__ gy, double *__restrict__ gz,
+		double *__restrict__ cost)
+{
+	const int nact = *nact_dev;
+	const int ngroups = (nact + 63) / 64;
+	const int nparts = walk32_parts(ngroups, split);
+	const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+	if (nparts == 1 || gid >= nact)
+		return;
+	const size_t nslot = (size_t)ngroups * 64;
+	double4 a = partial[gid];
+	for (int k = 1; k < nparts; k++) {
+		const double4 b = partial[k * nslot + gid];
+		a.x += b.x;
+		a.y += b.y;
+		a.z += b.z;
+		a.w += b.w;
+	}
+	const int ip = active[gid];
+	ax[ip] = ax[ip] + a.x;
+	ay[ip] = ay[ip] + a.y;
+	az[ip] = az[ip] + a.z;
+	gx[ip] = a.x;
+	gy[ip] = a.y;
+	gz[ip] = a.z;
+	cost[ip] = a.w;
 }
 
 __global__ void k_walk32_queue_reset(unsigned *__restrict__ queue,
@@ -773,6 +975,9 @@ int tdv_walk32(tdv_context *c, int use_bh)
 		TDV_HIP(c, hipMalloc((void **)&c->p32, c->cap * sizeof(int4)));
 		TDV_HIP(c, hipMalloc((void **)&c->p32o, c->cap * sizeof(float4)));
 	}
+	if (!c->partial)                     /* split walks' partial sums */
+		TDV_HIP(c, hipMalloc((void **)&c->partial,
+				(size_t)TDV_SPLIT_ITEMS * 64 * sizeof(double4)));
 	if (c->n32_cap < c->node_cap) {
 		tdv_graph_drop(c);               /* the records move */
 		if (c->a32)
@@ -781,6 +986,12 @@ int tdv_walk32(tdv_context *c, int use_bh)
 			(void)hipFree(c->b32);
 		if (c->q32)
 			(void)hipFree(c->q32);
+		if (c->q_s6)
+			(void)hipFree(c->q_s6);
+		if (c->q_arrived)
+			(void)hipFree(c->q_arrived);
+		c->q_s6 = nullptr;
+		c->q_arrived = nullptr;
 		c->a32 = c->b32 = nullptr;
 		c->q32 = nullptr;
 		c->n32_cap = 0;
@@ -788,6 +999,10 @@ int tdv_walk32(tdv_context *c, int use_bh)
 		TDV_HIP(c, hipMalloc((void **)&c->b32, c->node_cap * sizeof(int4)));
 		TDV_HIP(c, hipMalloc((void **)&c->q32,
 				2 * c->node_cap * sizeof(float4)));
+		TDV_HIP(c, hipMalloc((void **)&c->q_s6,
+				6 * c->node_cap * sizeof(double)));
+		TDV_HIP(c, hipMalloc((void **)&c->q_arrived,
+				c->node_cap * sizeof(unsigned)));
 		c->n32_cap = c->node_cap;
 	}
 	const bool quad = c->par.fp32_order != 1;
@@ -810,13 +1025,16 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	/* quadrupoles: masses G S^2, lengths S -> scale G S^4 */
 	const double qscale = mscale * S * S;
 	if (quad) {
-		hipLaunchKernelGGL(k_prep32_top_quad,
-				dim3(tdv_blocks((size_t)c->ntop, 4)), dim3(256), 0, c->stream,
-				c->ntop, c->top, c->pm, qscale, c->tq32);
-		if (nn)
-			hipLaunchKernelGGL(k_prep32_quad, dim3(tdv_blocks(nn, 4)),
-					dim3(256), 0, c->stream, (int)nn, c->node_c, c->node_a,
-					c->pm, qscale, c->q32);
+		if (nn) {
+			hipLaunchKernelGGL(k_moments_clear, dim3(tdv_blocks(nn, 256)),
+					dim3(256), 0, c->stream, (int)nn, c->q_arrived);
+			hipLaunchKernelGGL(k_moments_up, dim3(tdv_blocks(nn, 256)),
+					dim3(256), 0, c->stream, (int)nn, c->node_c, c->node_up,
+					c->node_a, c->pm, c->q_s6, c->q_arrived, qscale, c->q32);
+		}
+		hipLaunchKernelGGL(k_moments_top, dim3(tdv_blocks((size_t)c->ntop,
+				64)), dim3(64), 0, c->stream, c->ntop, c->top,
+				c->par.vector_size, c->node_a, c->pm, c->q_s6, qscale, c->tq32);
 	}
 	hipLaunchKernelGGL(k_prep32_top, dim3(tdv_blocks((size_t)c->ntop, 64)),
 			dim3(64), 0, c->stream, c->ntop, c->top, o[0], o[1], o[2], S,
@@ -880,17 +1098,27 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	w.carry_every = c->par.fp32_carry > 0 ? c->par.fp32_carry : 16;
 	w.quad = quad ? 1 : 0;
 	w.crit3 = c->par.fp32_criterion == 3;
-	if (c->par.fp32_queue || c->capturing) {
+	const int split = c->par.fp32_split;
+	const bool parts = walk32_parts((int)((c->nactive + 63) / 64), split) > 1;
+	if (c->par.fp32_queue || c->capturing || parts) {
 		/* grid: enough waves to fill every SIMD at the 8-wave budget */
 		hipLaunchKernelGGL(k_walk32_queue_reset, dim3(1), dim3(64), 0,
 				c->stream, c->d_queue, c->d_count);
 		hipLaunchKernelGGL(k_walk32_queue, dim3(TDV_QUEUE_GRID), dim3(64), 0,
-				c->stream, c->d_nactive, c->d_queue, 0, c->active, c->p32,
-				c->last_acc, c->top, c->a32, c->b32, c->node_c, c->ewald16, w,
-				c->acc[0], c->acc[1], c->acc[2], c->gacc[0], c->gacc[1],
-				c->gacc[2], c->cost, c->d_count, c->top32, c->q32, c->tq32,
-				c->p32o);
-		return tdv_check(c, hipGetLastError(), "k_walk32_queue");
+				c->stream, c->d_nactive, c->d_queue, split, c->partial, 0,
+				c->active, c->p32, c->last_acc, c->top, c->a32, c->b32,
+				c->node_c, c->ewald16, w, c->acc[0], c->acc[1], c->acc[2],
+				c->gacc[0], c->gacc[1], c->gacc[2], c->cost, c->d_count,
+				c->top32, c->q32, c->tq32, c->p32o);
+		TDV_TRY(tdv_check(c, hipGetLastError(), "k_walk32_queue"));
+		/* a captured walk serves every active count: always sum (the
+		 * kernel returns at once when the walk did not split) */
+		if (parts || c->capturing)
+			hipLaunchKernelGGL(k_walk32_parts_sum, dim3(tdv_blocks(n, 256)),
+					dim3(256), 0, c->stream, c->d_nactive, split, c->active,
+					c->partial, c->acc[0], c->acc[1], c->acc[2], c->gacc[0],
+					c->gacc[1], c->gacc[2], c->cost);
+		return tdv_check(c, hipGetLastError(), "k_walk32_parts_sum");
 	}
 	auto kern = c->par.fp32_waves == 6 ? k_walk32_w6
 			: c->par.fp32_waves == 8 ? k_walk32_w8 : k_walk32;

@@ -40,7 +40,8 @@ class TdvParams(ctypes.Structure):
                 ("fp32_criterion", ctypes.c_int),
                 ("fp32_carry", ctypes.c_int),
                 ("fp32_waves", ctypes.c_int),
-                ("fp32_queue", ctypes.c_int)]
+                ("fp32_queue", ctypes.c_int),
+                ("fp32_split", ctypes.c_int)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -105,6 +106,7 @@ def load_library(path=LIB_PATH):
     L.tdv_set_active.argtypes = [_ctx, ctypes.c_int, _vp]
     L.tdv_set_tree_moments.argtypes = [_ctx, _vp, _sz, _vp, ctypes.c_int]
     L.tdv_get_ewald_derivs.argtypes = [_ctx, _dp]
+    L.tdv_get_walk32_quadrupoles.argtypes = [_ctx, _vp, _vp, _vp]
     L.tdv_capture_walk.argtypes = [_ctx]
     L.tdv_launch_walk.argtypes = [_ctx]
     L.tdv_num_particles.argtypes = [_ctx]
@@ -132,7 +134,7 @@ class Engine:
                  compat=True, device=0, ewald_table=None, precision="fp64",
                  ewald_size=0.125, ewald_accuracy=0.1, fp32_order=2,
                  fp32_criterion=2, fp32_carry=16, fp32_waves=0,
-                 fp32_queue=0):
+                 fp32_queue=0, fp32_split=0):
         self.lib = load_library()
         prec = {"fp64": 0, "fp32": 1}[precision]
         self.params = TdvParams(int(periodic), float(boxsize), float(gravity),
@@ -142,7 +144,7 @@ class Engine:
                                 float(ewald_size), float(ewald_accuracy),
                                 int(fp32_order), int(fp32_criterion),
                                 int(fp32_carry), int(fp32_waves),
-                                int(fp32_queue))
+                                int(fp32_queue), int(fp32_split))
         self._c = _ctx()
         rc = self.lib.tdv_create(ctypes.byref(self.params),
                                  ctypes.byref(self._c))
@@ -444,6 +446,20 @@ class Engine:
                                        n.value, ctypes.byref(n)),
                  "tdv_get_tree")
         return buf
+
+    def walk32_quadrupoles(self):
+        """The FP32 walk's quadrupole records of the last FP32 walk: (nodes
+        (nnodes, 8) float32, top nodes (ntop, 8) float32, each node's first
+        particle (-1 for terminators))."""
+        nn = len(self.tree())
+        ntop = len(self.top_nodes())
+        q = np.zeros((nn, 8), np.float32)
+        tq = np.zeros((ntop, 8), np.float32)
+        first = np.zeros(nn, np.int32)
+        self._ok(self.lib.tdv_get_walk32_quadrupoles(
+            self._c, q.ctypes.data_as(_vp), tq.ctypes.data_as(_vp),
+            first.ctypes.data_as(_vp)), "tdv_get_walk32_quadrupoles")
+        return q, tq, first
 
     def interaction_counts(self):
         """(tree-walk interactions, Ewald interactions) of the last walk."""

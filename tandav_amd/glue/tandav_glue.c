@@ -543,14 +543,6 @@ void Compute_Acceleration(void)
 	if (Sig.Tree_Update)
 		Gravity_Tree_Build();
 
-	/* the momentum check brackets each walk (one, or BH + relative on the
-	 * prepare step); velocities do not change in between */
-	const int nwalks = Sig.Prepare_Step ? 2 : 1;
-	for (int w = 0; w < nwalks; w++) {
-		rprintf("Tree acceleration ");
-		check_total_momentum(false);
-	}
-
 	#pragma omp single
 	{
 	tdv_context *c = context();
@@ -558,7 +550,14 @@ void Compute_Acceleration(void)
 	download_forces(c, true);
 	} // omp single
 
+	/* Gravity_Tree_Acceleration's output and momentum check for each walk
+	 * (BH + relative on the prepare step), in Tandav's order: the walks do
+	 * not change velocities, so running the checks after the device work
+	 * prints the same lines and values */
+	const int nwalks = Sig.Prepare_Step ? 2 : 1;
 	for (int w = 0; w < nwalks; w++) {
+		rprintf("Tree acceleration ");
+		check_total_momentum(false);
 		rprintf(" done \n");
 		check_total_momentum(true);
 	}

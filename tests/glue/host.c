@@ -31,7 +31,7 @@ void host_set_verbose(int on)
 {
 	Host_Verbose = on != 0;
 	#pragma omp parallel num_threads(Threads)
-	Task.Is_Master = Host_Verbose;
+	Task.Is_Master = Host_Verbose && omp_get_thread_num() == 0;
 }
 static int Max_Active_Bin = 0;
 static size_t N = 0;
@@ -66,8 +66,8 @@ static void set_task(void)
 	Task.Npart_Total = (int)N;
 	memset(Task.Npart, 0, sizeof(Task.Npart));
 	Task.Npart[1] = (int)N;           /* every particle is type 1 */
-	Task.Is_Master = Host_Verbose;
 	Task.Thread_ID = omp_get_thread_num();
+	Task.Is_Master = Host_Verbose && Task.Thread_ID == 0;   /* main.c:139 */
 	Task.Seed[0] = 11;
 	Task.Seed[1] = 22;
 	Task.Seed[2] = 33;

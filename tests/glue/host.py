@@ -30,11 +30,15 @@ def _d(a):
     return a.ctypes.data_as(_dp)
 
 
-def load(periodic):
-    path = build_host.lib_path(periodic)
-    if not os.path.exists(path):
-        build_host.build()
+def load(periodic, path=None):
+    """The host library; `path` loads a copy of it, which gets its own
+    globals and statics (a fresh program)."""
+    if path is None:
+        path = build_host.lib_path(periodic)
+        if not os.path.exists(path):
+            build_host.build()
     L = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
+    L.host_set_verbose.argtypes = [ctypes.c_int]
     L.host_init.argtypes = [ctypes.c_int] + [_dp] * 7 + [_vp] + \
         [ctypes.c_double] * 5 + [ctypes.c_int]
     L.host_set_domain.argtypes = [ctypes.c_double, _dp]
@@ -71,8 +75,8 @@ _TYPES = {"Type": np.int32, "Time_Bin": np.int32, "It_Drift_Pos": np.uint32,
 class Host:
     """One host program (one process-wide set of Tandav globals)."""
 
-    def __init__(self, periodic):
-        self.L = load(periodic)
+    def __init__(self, periodic, path=None):
+        self.L = load(periodic, path)
         self.periodic = periodic
         self.n = 0
 

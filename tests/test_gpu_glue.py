@@ -176,3 +176,35 @@ def test_binding_element_functions():
     want = np.array([s.ewald_correction(d) for d in dr])
     np.testing.assert_allclose(h.ewald_correction(dr), want, rtol=0,
                                atol=1e-11)
+
+
+def test_binding_momentum_check_output(tmp_path, capfd):
+    """check_total_momentum (Gravity/tree_accel.c:442-469) brackets each
+    walk: its running sums are never reset, so a fresh program prints the
+    relative changes 1 and 1/3 on the prepare step (BH walk, then relative
+    walk) and 1/5 on the next step, whatever the velocities."""
+    import shutil
+    fresh = tmp_path / "libtandav_host_fresh.so"
+    shutil.copy(glue_host.build_host.lib_path(False), fresh)
+    pos, vel, m = ics.hernquist_sphere(4096, seed=5)
+    e = Engine(periodic=False, softening=0.01, alpha_rel=0.005)
+    e.set_particles(pos, m, vel=vel)
+    e.setup_domain()
+    top = _engine_step(e, True)
+    h = glue_host.Host(False, path=str(fresh))
+    h.init(pos, vel, m, softening=0.01, alpha_rel=0.005)
+    h.L.host_set_max_active_bin(10)
+    h.L.host_set_verbose(1)
+    size, origin, _ = e.domain
+    h.decomposition(size, origin, top)
+    capfd.readouterr()
+    h.L.host_compute_acceleration(1)
+    h.L.host_compute_acceleration(0)
+    import ctypes
+    ctypes.CDLL(None).fflush(None)
+    out = capfd.readouterr().out
+    h.L.host_set_verbose(0)
+    vals = [float(l.split(":")[1]) for l in out.splitlines()
+            if l.startswith("Total change in momentum due to gravity:")]
+    assert vals == pytest.approx([1.0, 1 / 3, 1 / 5], rel=1e-5), out
+    assert out.count("Tree acceleration  done") == 3, out

@@ -1,7 +1,9 @@
 """The persistent work-queue walk and the captured walk graph against the
 plain FP32 walk: the same groups run the same code, so the results must be
 bit-identical, on every replay, with eager work and synchronisations between
-the replays, for the full active set and for a subset."""
+the replays, for the full active set and for subsets (which split each
+group's walk over several waves).  The split walk against the whole walk:
+the same interactions (Cost and counts equal) summed in another order."""
 import numpy as np
 import pytest
 
@@ -10,11 +12,11 @@ from tandav_amd import Engine, ics
 pytestmark = pytest.mark.gpu
 
 
-def _engine(pos, m, queue):
-    e = Engine(periodic=True, softening=1 / 1280., alpha_rel=0.01,
+def _engine(pos, m, queue, split=0, periodic=True, softening=1 / 1280.):
+    e = Engine(periodic=periodic, softening=softening, alpha_rel=0.01,
                precision="fp32", compat=False, ewald_size=0.125,
                ewald_accuracy=1.0, fp32_order=2, fp32_criterion=3,
-               fp32_queue=queue)
+               fp32_queue=queue, fp32_split=split)
     e.set_particles(pos, m)
     e.setup_domain()
     e.domain_decomposition()
@@ -25,8 +27,8 @@ def _engine(pos, m, queue):
 
 def test_queue_walk_matches_grid_walk():
     pos, _, m = ics.zeldovich_box(32, seed=7)
-    a = _engine(pos, m, 0)
-    b = _engine(pos, m, 1)
+    a = _engine(pos, m, 0, split=1)
+    b = _engine(pos, m, 1, split=1)
     ga, gb = a.get("acc", "cost"), b.get("acc", "cost")
     np.testing.assert_array_equal(ga["acc"], gb["acc"])
     np.testing.assert_array_equal(ga["cost"], gb["cost"])
@@ -62,3 +64,42 @@ def test_graph_replays_match_eager_walk():
             # eager work between replays
             e.get("pos")
             e.forcetest([0, 1])
+
+
+@pytest.mark.parametrize("case", ["zeldovich", "clustered", "hernquist"])
+def test_split_walk_matches_whole_walk(case):
+    if case == "zeldovich":
+        pos, _, m = ics.zeldovich_box(32, seed=7)
+        kw = {}
+    elif case == "clustered":
+        pos, _, m = ics.clustered_box(1 << 16, nhalo=32, seed=11)
+        kw = dict(softening=1 / 2560.)
+    else:
+        pos, _, m = ics.hernquist_sphere(30000, seed=4)
+        kw = dict(periodic=False, softening=0.01)
+    whole = _engine(pos, m, 0, split=1, **kw)
+    n = len(m)
+    g = np.random.Generator(np.random.Philox(5))
+    # the full set (auto: 16384 / groups parts), a forced 3-way split, and
+    # a few groups (64 parts)
+    ref = whole.get("pos", "last_acc_mag")
+    zero = np.zeros((n, 3))
+    for split, sub in ((0, np.arange(n)), (3, np.arange(n)),
+                       (0, np.sort(g.choice(n, 150, replace=False)))):
+        parts = _engine(pos, m, 0, split=split, **kw)
+        for e in (whole, parts):         # the same criterion input
+            e.set_active(sub)
+            e.update_particles(ref["pos"], acc=zero,
+                               last_acc_mag=ref["last_acc_mag"])
+            e.tree_acceleration(False)
+        a, b = whole.get("acc", "cost"), parts.get("acc", "cost")
+        np.testing.assert_array_equal(a["cost"][sub], b["cost"][sub])
+        assert whole.interaction_counts() == parts.interaction_counts()
+        amag = np.linalg.norm(a["acc"][sub], axis=1)
+        err = np.linalg.norm(a["acc"][sub] - b["acc"][sub], axis=1) / amag
+        # the same interactions summed in another order: FP32 rounding of
+        # partial sums, largest where |a| is a small residue of large terms
+        # (the near-uniform box), far below the walk's 2e-3 error bar
+        assert np.median(err) < 1e-5, (case, split, len(sub), np.median(err))
+        assert err.max() < 1e-4, (case, split, len(sub), err.max())
+        parts.close()
