@@ -308,6 +308,8 @@ int tdv_create(const tdv_params *p, tdv_context **out)
 	if (!r)
 		r = dalloc(c, &c->d_queue, 1);
 	if (!r)
+		r = dalloc(c, &c->long_count, 2);
+	if (!r)
 		r = tdv_check(c, hipMemset(c->d_err, 0, sizeof(unsigned)), "memset");
 	if (!r)
 		r = alloc_top(c, 64);
@@ -352,6 +354,8 @@ void tdv_destroy(tdv_context *c)
 	dfree(c->d_nactive);
 	dfree(c->d_queue);
 	dfree(c->partial);
+	dfree(c->long_count);
+	dfree(c->long_list);
 	for (int e = 0; e < EV_N; e++)
 		if (c->ev[e])
 			(void)hipEventDestroy(c->ev[e]);

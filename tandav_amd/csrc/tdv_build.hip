@@ -265,7 +265,8 @@ __global__ void __launch_bounds__(256) k_tree_nodes(size_t n,
 		const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ off,
 		const int *__restrict__ lohi, double4 *__restrict__ na,
 		int4 *__restrict__ nc, int *__restrict__ up,
-		int *__restrict__ tree_parent)
+		int *__restrict__ tree_parent, int2 *__restrict__ long_list,
+		unsigned long_cap, unsigned *__restrict__ long_count, unsigned *err)
 {
 	const size_t gi = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (gi >= n)
@@ -277,6 +278,27 @@ __global__ void __launch_bounds__(256) k_tree_nodes(size_t n,
 	const int lo = lohi[i] & 0xFF, hi = (lohi[i] >> 8) & 0xFF;
 	const int base = node_index(T, r.t, r.first, off, i, r.last);
 	const Key2 ki = load_key(key, i);
+	{
+		/* the chain's scan covers the level-lo cell starting at i: its last
+		 * particle is the last k with div_level(key_i, key_k) > lo (keys are
+		 * sorted, so that level only falls with k) */
+		int a = i, b = r.last;
+		while (a < b) {
+			const int mid = a + ((b - a + 1) >> 1);
+			if (div_level(ki, load_key(key, mid)) > lo)
+				a = mid;
+			else
+				b = mid - 1;
+		}
+		if (a - i + 1 > TDV_LONG_CHAIN) {    /* k_tree_nodes_long's */
+			const unsigned slot = atomicAdd(long_count, 1u);
+			if (slot < long_cap)
+				long_list[slot] = make_int2(i, a);
+			else
+				atomicOr(err, TDV_EBIT_RANGE);
+			return;
+		}
+	}
 
 	double sx = 0, sy = 0, sz = 0, sm = 0;
 	int cur = hi;
@@ -335,6 +357,141 @@ __global__ void __launch_bounds__(256) k_tree_nodes(size_t n,
 			runmin = dn;
 		kprev = kk;
 		k = kn;
+	}
+}
+
+/* The chains whose scan covers more than TDV_LONG_CHAIN particles (the
+ * roots and upper nodes of large top nodes), one wavefront per chain, taken
+ * from a work queue.  The result is k_tree_nodes's, bit for bit: the lanes
+ * load 64 particles at a time (coalesced), form m x, m y, m z and the
+ * divergence level of each particle with the next, and set the parent links
+ * of the chains that start inside this one from a running minimum of those
+ * levels (a prefix minimum over the lanes); the FP64 sums then run over the
+ * particles in PH order from LDS, every lane keeping the same sums, and the
+ * cells that end at a particle (its level <= the deepest open level, found
+ * by a ballot) take the sums at that particle. */
+__global__ void __launch_bounds__(64) k_tree_nodes_long(
+		const uint64_t *__restrict__ key, const double *__restrict__ x,
+		const double *__restrict__ y, const double *__restrict__ z,
+		const double *__restrict__ m, TopNodes T, int vs,
+		const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ off,
+		const int *__restrict__ lohi, double4 *__restrict__ na,
+		int4 *__restrict__ nc, int *__restrict__ up,
+		int *__restrict__ tree_parent, const int2 *__restrict__ long_list,
+		const unsigned *__restrict__ long_count, unsigned long_cap,
+		unsigned *__restrict__ queue)
+{
+	__shared__ double4 sh[64];
+	__shared__ int shdn[64];
+	const int lane = threadIdx.x;
+	/* an overflow is flagged by k_tree_nodes; read no further than the list */
+	const unsigned nl = *long_count;
+	const int nlong = (int)(nl < long_cap ? nl : long_cap);
+	for (;;) {
+		/* wave-uniform ticket, as in the walk's queue */
+		const unsigned tk = atomicAdd(queue, lane == 0 ? 1u : 0u);
+		const int item = __builtin_amdgcn_readfirstlane((int)tk);
+		if (item >= nlong)
+			break;
+		const int2 le = long_list[item];
+		const int i = le.x, e = le.y;
+		const TopRange r = top_of(key, (size_t)i, T, vs);
+		const int lo = lohi[i] & 0xFF, hi = (lohi[i] >> 8) & 0xFF;
+		const int base = node_index(T, r.t, r.first, off, i, r.last);
+		const Key2 ki = load_key(key, i);
+		double sx = 0, sy = 0, sz = 0, sm = 0;
+		int cur = hi, runmin = 99;
+		for (int b0 = i; b0 <= e && cur >= lo; b0 += 64) {
+			const int k = b0 + lane;
+			const bool v = k <= e;
+			double tx = 0, ty = 0, tz = 0, mk = 0;
+			int dn = 99;
+			if (v) {
+				mk = m[k];
+				tx = x[k] * mk;
+				ty = y[k] * mk;
+				tz = z[k] * mk;
+				dn = k + 1 <= r.last ? div_level(load_key(key, k),
+						load_key(key, k + 1)) : -1;
+			}
+			/* runmin before particle k: min of the levels of i .. k-1 */
+			int incl = dn;
+			for (int d = 1; d < 64; d <<= 1) {
+				const int o = __shfl_up(incl, d, 64);
+				if (lane >= d)
+					incl = o < incl ? o : incl;
+			}
+			int excl = __shfl_up(incl, 1, 64);
+			if (lane == 0)
+				excl = 99;
+			const int rm = runmin < excl ? runmin : excl;
+			/* not at e: the chain's last particle closes it before the
+			 * single-lane scan looks for a chain starting after it */
+			if (v && k < e && k + 1 <= r.last && rm >= dn && cnt[k + 1] > 0) {
+				const int jk = node_index(T, r.t, r.first, off, k + 1, r.last);
+				const int jp = base + (dn - 1 - lo);
+				up[jk] = jk - jp;
+			}
+			const int bmin = __shfl(incl, 63, 64);
+			runmin = runmin < bmin ? runmin : bmin;
+			sh[lane] = make_double4(tx, ty, tz, mk);
+			shdn[lane] = dn;
+			__syncthreads();
+			const int nb = e - b0 + 1 < 64 ? e - b0 + 1 : 64;
+			int q0 = 0;
+			while (q0 < nb) {
+				/* the next particle at which open cells end */
+				const unsigned long long ends = __ballot(lane >= q0
+						&& lane < nb && dn <= cur);
+				const int qe = ends ? __ffsll((long long)ends) - 1 : nb - 1;
+				for (int q = q0; q <= qe; q++) {
+					const double4 t = sh[q];
+					sx = sx + t.x;
+					sy = sy + t.y;
+					sz = sz + t.z;
+					sm = sm + t.w;
+				}
+				q0 = qe + 1;
+				if (!ends)
+					break;
+				const int kn = b0 + qe + 1;
+				const int dnq = shdn[qe];
+				const int lowest = dnq > lo ? dnq : lo;
+				const int next = node_index(T, r.t, r.first, off, kn, r.last);
+				for (int lv = lowest + lane; lv <= cur; lv += 64) {
+					const int j = base + (lv - lo);
+					const bool leaf = lv == hi;
+					int bits = lv | ((int)key_triplet(ki, lv) << 6)
+							| (1 << TDV_BIT_LOCAL);
+					const bool root = j == T.target[r.t];
+					if (root)
+						bits |= 1 << TDV_BIT_TOP;
+					const int dnext = leaf ? (-i - 1) : (next - j);
+					na[j] = make_double4(sx / sm, sy / sm, sz / sm, sm);
+					nc[j] = make_int4(dnext, kn - i, bits, i);
+					if (root)
+						up[j] = r.t;
+					else if (lv > lo)
+						up[j] = 1;
+				}
+				if (cur == hi)            /* the leaf: its particles */
+					for (int q = i + lane; q < kn; q += 64)
+						tree_parent[q] = base + (hi - lo);
+				cur = lowest - 1;
+				if (cur < lo)
+					break;
+			}
+			__syncthreads();
+		}
+	}
+}
+
+__global__ void k_tree_long_reset(unsigned *__restrict__ long_count,
+		unsigned *__restrict__ queue)
+{
+	if (threadIdx.x == 0) {
+		long_count[0] = 0;
+		queue[0] = 0;
 	}
 }
 
@@ -463,10 +620,28 @@ int tdv_build_tree(tdv_context *c)
 	if (nn > 0)
 		TDV_HIP(c, hipMemsetAsync(c->node_c, 0, (size_t)nn * sizeof(int4),
 				c->stream));
+	/* at most n / (TDV_LONG_CHAIN + 1) disjoint long cells per level */
+	const size_t long_need = (n / (TDV_LONG_CHAIN + 1) + 1) * (TDV_NT + 2);
+	if (c->long_cap < long_need) {
+		if (c->long_list)
+			(void)hipFree(c->long_list);
+		c->long_list = nullptr;
+		c->long_cap = 0;
+		TDV_HIP(c, hipMalloc((void **)&c->long_list, long_need * sizeof(int2)));
+		c->long_cap = long_need;
+	}
+	hipLaunchKernelGGL(k_tree_long_reset, dim3(1), dim3(64), 0, c->stream,
+			c->long_count, c->long_count + 1);
 	hipLaunchKernelGGL(k_tree_nodes, dim3(tdv_blocks(n, 256)), dim3(256), 0,
 			c->stream, n, c->key, c->pos[0], c->pos[1], c->pos[2], c->mass,
 			c->top, vs, reinterpret_cast<uint32_t *>(c->cnt), c->off, c->lohi,
-			c->node_a, c->node_c, c->node_up, c->tree_parent);
+			c->node_a, c->node_c, c->node_up, c->tree_parent, c->long_list,
+			(unsigned)c->long_cap, c->long_count, c->d_err);
+	hipLaunchKernelGGL(k_tree_nodes_long, dim3(TDV_LONG_GRID), dim3(64), 0,
+			c->stream, c->key, c->pos[0], c->pos[1], c->pos[2], c->mass,
+			c->top, vs, reinterpret_cast<uint32_t *>(c->cnt), c->off, c->lohi,
+			c->node_a, c->node_c, c->node_up, c->tree_parent, c->long_list,
+			c->long_count, (unsigned)c->long_cap, c->long_count + 1);
 	hipLaunchKernelGGL(k_tree_terms, dim3(tdv_blocks((size_t)c->ntop, 64)),
 			dim3(64), 0, c->stream, c->top, vs, c->node_a, c->node_b, c->node_c,
 			c->node_up, c->ntop);

@@ -85,6 +85,9 @@ struct tdv_context {
 
 	/* build scratch (per particle) */
 	uint32_t *minpack;
+	int2 *long_list;                 /* build: chains scanned by a wave */
+	size_t long_cap;
+	unsigned *long_count;            /* [0] long chains, [1] their queue */
 	int *cnt;
 	uint32_t *off;                   /* n+1 exclusive scan of cnt */
 	int *lohi;
@@ -160,6 +163,9 @@ void tdv_graph_drop(tdv_context *c);
                                             start later and find the queue
                                             drained */
 #define TDV_SPLIT_ITEMS 16384            /* work items a split walk aims at */
+#define TDV_LONG_CHAIN 2048              /* build: longer chain scans go to
+                                            one wavefront each */
+#define TDV_LONG_GRID 2048               /* waves of that kernel */
 #define TDV_SPLIT_AUTO_MAX 64            /* parts per group, split 0 */
 int tdv_direct_sum(tdv_context *c, int nt, const int *idx_dev, double *out_dev);
 int tdv_direct_sum_exact(tdv_context *c, int nt, const int *idx_dev,

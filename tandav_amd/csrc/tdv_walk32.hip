@@ -35,6 +35,7 @@ struct Walk32Params {
 	float theta_bh, alpha_rel;
 	float eps2, inv_eps2, k_inner;   /* fixed units: eps^2, 1/eps^2,
 	                                    1/(16 eps^3) */
+	float eps_abs;                   /* |eps|, fixed units */
 	float ewald_size;                /* correction at nodes <= this size */
 	float ewald_acc;                 /* factor on the criterion for that */
 	float half;                      /* half box in fixed units */
@@ -555,10 +556,16 @@ __device__ __forceinline__ bool ewald_ok(float m, float s, float rx, float ry,
 
 /* The walk's decision at an internal node (tree_accel.c:249-262): the BH
  * criterion, or the relative one or the particle within 0.6 s of the
- * node's centre; (rx, ry, rz) to the CoM, (sx, sy, sz) to the centre. */
+ * node's centre; (rx, ry, rz) to the CoM, (sx, sy, sz) to the centre.  A
+ * node whose particles may lie inside the softening length (r < |eps| + s)
+ * is opened too: there the force is not Newtonian and the node's multipole
+ * expansion does not hold. */
 __device__ __forceinline__ bool node_opens(float m, float s, float r2,
 		float sx, float sy, float sz, float crit, const Walk32Params &w)
 {
+	const float es = w.eps_abs + s;
+	if (r2 < es * es)
+		return true;
 	if (w.use_bh)
 		return s * s > r2 * w.theta_bh;
 	return rel_open(m, s, r2, crit, w) || (fabsf(sx) < 0.6f * s
@@ -1077,6 +1084,7 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	if (ef * ef < 0.5)
 		ef = -sqrt(0.5);
 	w.eps2 = (float)(ef * ef);
+	w.eps_abs = (float)fabs(ef);
 	w.inv_eps2 = (float)(1.0 / (ef * ef));
 	w.k_inner = (float)(1.0 / (16.0 * ef * ef * ef));
 	w.box_f = (float)(c->par.periodic ? 4294967296.0 : 2147483648.0);
