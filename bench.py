@@ -273,15 +273,140 @@ def bench_c4(args, f32):
     print(json.dumps(line), flush=True)
 
 
+# C5: the 512^3 box of the Zel'dovich family (P(k) ~ k^-2.5, seed 7) at rms
+# |psi| = 0.5 grid spacing, one full step at a = 0.1 of a flat LCDM run
+# (Omega_m 0.3): keys, sort, build, FP32 walk + Ewald, comoving kick-drift.
+# Units: box 1, total mass 1, G 1, so H0 = sqrt(8 pi / (3 Omega_m)).
+C5 = dict(ngrid=512, seed=7, rms=0.5, a=0.1, dlna=0.01, omega_m=0.3,
+          omega_l=0.7,
+          desc="periodic box, Zel'dovich P(k) ~ k^-2.5 (seed 7) at rms |psi| "
+          "0.5 grid spacing, 512^3 = 134,217,728 particles, full step at "
+          "a = 0.1 (flat LCDM, Omega_m 0.3, dln a 0.01)")
+
+
+def bench_c5(args, f32):
+    """One full step of the large box with every particle active, device
+    resident: PH keys, radix sort, reorder, top nodes, build, the FP32 walk
+    with its Ewald correction, then the comoving kick (the step's two half
+    kicks) and drift.  The step is run twice from the same positions: the
+    first, untimed, is scored against the oracles (the table direct sum of
+    Gravity_Forcetest on --c5-sample particles and the exact Ewald sums on
+    --exact-sample), the second is timed with the kick-drift."""
+    import torch
+    from tandav_amd import Engine, ics
+    from tandav_amd.comoving import ComovingFactors, e_hubble
+    ng = args.c5_ngrid or C5["ngrid"]
+    n = ng ** 3
+    a0 = C5["a"]
+    a1 = a0 * np.exp(C5["dlna"])
+    om, ol = C5["omega_m"], C5["omega_l"]
+    h0 = np.sqrt(8.0 * np.pi / (3.0 * om))
+    hub = h0 * e_hubble(a0, om + ol, om, ol)
+    om_a = om / (a0 ** 3 * (hub / h0) ** 2)
+    vel_factor = a0 * hub * om_a ** 0.55          # comoving a H f
+    log("c5: generating %d^3 Zel'dovich box" % ng)
+    t0 = time.perf_counter()
+    pos, vel, m = ics.zeldovich_box(ng, seed=C5["seed"], rms_frac=C5["rms"],
+                                    vel_factor=vel_factor)
+    log("c5: initial conditions %.1fs" % (time.perf_counter() - t0))
+    fac = ComovingFactors(0.9 * a0, 1.1 * a1, hubble=h0, omega_matter=om,
+                          omega_lambda=ol)
+    kick, drift = fac.kick(a0, a1), fac.drift(a0, a1)
+    soft = 1.0 / (40.0 * ng)
+    free0 = torch.cuda.mem_get_info()[0]
+    e = Engine(periodic=True, boxsize=1.0, softening=soft, theta_bh=0.1,
+               alpha_rel=f32["alpha"], compat=False, precision="fp32",
+               ewald_size=f32["ewald_size"],
+               ewald_accuracy=f32["ewald_accuracy"],
+               fp32_order=f32["fp32_order"],
+               fp32_criterion=f32["fp32_criterion"])
+    e.set_particles(pos, m, vel=vel)
+    del pos, vel
+    e.setup_domain()
+    e.domain_decomposition()
+    log("c5: prepare step")
+    t0 = time.perf_counter()
+    e.compute_acceleration(prepare_step=True)
+    log("c5: prepare step %.1fs" % (time.perf_counter() - t0))
+    e.domain_decomposition()
+    e.compute_acceleration(prepare_step=False)
+    log("c5: scored step done")
+    acc = e.get("acc")["acc"]
+    g = np.random.Generator(np.random.Philox(5))
+    idx = np.sort(g.choice(n, min(n, args.c5_sample), replace=False))
+    t0 = time.perf_counter()
+    ref = e.forcetest(idx)
+    log("c5: table direct sum on %d particles %.1fs"
+        % (len(idx), time.perf_counter() - t0))
+    xidx = idx[np.sort(np.random.Generator(np.random.Philox(6)).choice(
+        len(idx), min(len(idx), args.c5_exact_sample), replace=False))]
+    t0 = time.perf_counter()
+    exact = e.forcetest_exact(xidx)
+    log("c5: exact Ewald sums on %d particles %.1fs"
+        % (len(xidx), time.perf_counter() - t0))
+    err_table = errors(acc[idx], ref)
+    err_exact = errors(acc[xidx], exact)
+    pos_idx = {int(j): k for k, j in enumerate(idx)}
+    err_table_vs_exact = errors(ref[[pos_idx[int(j)] for j in xidx]], exact)
+    del acc
+    log("c5: timed step")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e.domain_decomposition()
+    e.compute_acceleration(prepare_step=False)
+    e.kick_drift(kick, drift)
+    t_step = time.perf_counter() - t0
+    stage = e.stage_times()
+    newton, ewald = e.interaction_counts()
+    free1 = torch.cuda.mem_get_info()[0]
+    walk_s = stage["walk"] / 1e3
+    flops = 20.0 * newton + 60.0 * ewald
+    line = {
+        "metric": "tree-force evals/s per full step (FP32 group walk), "
+                  "512^3 box, at p99 |da|/|a| <= 2e-3",
+        "value": n / t_step, "unit": "force-evals/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": t_step * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": C5["desc"] if ng == C5["ngrid"] else
+                   C5["desc"].replace("512^3 = 134,217,728",
+                                      "%d^3 = %d" % (ng, n)),
+                   "n_particles": n, "softening": soft,
+                   "fp32": f32, "kick": kick, "drift": drift,
+                   "parallelism": "single GPU"},
+        "stages_ms": {k: round(v, 3) for k, v in stage.items()},
+        "walk_force_evals_per_s": n / walk_s,
+        "interactions_per_s": (newton + ewald) / walk_s,
+        "interactions_per_particle": (newton + ewald) / n,
+        "walk_roofline": {"bound": "valu", "achieved": flops / walk_s / 1e12,
+                          "peak": 157.3, "unit": "TFLOP/s",
+                          "frac": flops / walk_s / 1e12 / 157.3,
+                          "flops_per_interaction": "20 Newton, 60 Ewald"},
+        "hbm_bytes_per_particle": (free0 - free1) / n,
+        "accuracy": {"vs_table_direct_sum": dict(err_table, n=len(idx)),
+                     "vs_exact_ewald": dict(err_exact, n=len(xidx)),
+                     "table_direct_sum_vs_exact": err_table_vs_exact},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2",
-                    choices=sorted(WORKLOADS) + ["c4"])
+                    choices=sorted(WORKLOADS) + ["c4", "c5"])
     ap.add_argument("--c4-n", type=int, default=0,
                     help="particles of the clustered box (default 2^23)")
+    ap.add_argument("--c5-ngrid", type=int, default=0,
+                    help="grid of the large box (default 512)")
+    ap.add_argument("--c5-sample", type=int, default=16384,
+                    help="particles scored against the table direct sum")
+    ap.add_argument("--c5-exact-sample", type=int, default=16,
+                    help="particles scored against the exact Ewald sums "
+                         "(about 4 s each at 512^3)")
     ap.add_argument("--fp32-alpha", type=float, default=None,
                     help="FP32 walk's relative opening criterion (default "
                          "%g, %g for the clustered box)"
@@ -329,6 +454,9 @@ def main():
                fp32_criterion=args.fp32_criterion)
     if args.workload == "c4":
         bench_c4(args, f32)
+        return
+    if args.workload == "c5":
+        bench_c5(args, f32)
         return
 
     log("generating %s" % args.workload)

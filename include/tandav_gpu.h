@@ -267,6 +267,16 @@ int tdv_set_active(tdv_context *ctx, int nactive, const int *list);
 int tdv_set_tree_moments(tdv_context *ctx, const void *tree, size_t nnodes,
 		const void *top, int ntop);
 
+/* Kick_First/Second_Halfstep (kick.c:9-70) for the active particles,
+ * v += kick * Acc, then Drift_To_Sync_Point (drift.c:12-64) for every
+ * particle, x += drift * v, wrapped into the box on periodic runs
+ * (Periodic_Constrain_Particles_To_Box, periodic.c:30-53), in FP64 on the
+ * device; a zero factor skips its half.  kick and drift are the caller's
+ * Particle_Kick_Step / Particle_Drift_Step (comoving.c:30-50).  The keys and
+ * tree stay: sort again (or move the tree's monopoles) before the next
+ * walk.  Timed as stage 7. */
+int tdv_kick_drift(tdv_context *ctx, double kick, double drift);
+
 /* Number of particles currently on the device. */
 size_t tdv_num_particles(const tdv_context *ctx);
 

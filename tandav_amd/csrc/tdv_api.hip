@@ -29,8 +29,6 @@ struct TopRecordIn {
 	float Pos[3], Mass, CoM[3], Dp[3];
 };
 
-enum { EV_KEYS0, EV_KEYS1, EV_SORT1, EV_REORDER1, EV_TOP1, EV_BUILD0,
-	EV_BUILD1, EV_WALK0, EV_WALK1, EV_EWALD1, EV_KICK0, EV_KICK1, EV_N };
 
 int tdv_fail(tdv_context *c, int code, const std::string &msg)
 {
@@ -310,6 +308,8 @@ int tdv_create(const tdv_params *p, tdv_context **out)
 	if (!r)
 		r = dalloc(c, &c->long_count, 2);
 	if (!r)
+		r = dalloc(c, &c->q_arrived, 192);
+	if (!r)
 		r = tdv_check(c, hipMemset(c->d_err, 0, sizeof(unsigned)), "memset");
 	if (!r)
 		r = alloc_top(c, 64);
@@ -346,6 +346,7 @@ void tdv_destroy(tdv_context *c)
 	dfree(c->q32);
 	dfree(c->q_s6);
 	dfree(c->q_arrived);
+	dfree(c->lvl_nodes);
 	dfree(c->tq32);
 	dfree(c->top32);
 	tdv_graph_drop(c);
@@ -716,7 +717,8 @@ int tdv_capture_walk(tdv_context *c)
 		return tdv_fail(c, TDV_ERR_ARG, "tdv_capture_walk: FP32 walk over a "
 				"built tree only");
 	if (!c->p32 || (c->par.periodic && !c->ewald16_ok)
-			|| c->n32_cap < c->node_cap || c->top32_cap < c->top_cap)
+			|| c->n32_cap < c->node_cap || c->top32_cap < c->top_cap
+			|| (c->par.fp32_order != 1 && c->nnodes && !c->lvl_ok))
 		return tdv_fail(c, TDV_ERR_ARG, "tdv_capture_walk: run one walk on "
 				"this tree first");
 	tdv_graph_drop(c);

@@ -399,7 +399,10 @@ __global__ void __launch_bounds__(64) k_tree_nodes_long(
 		const int lo = lohi[i] & 0xFF, hi = (lohi[i] >> 8) & 0xFF;
 		const int base = node_index(T, r.t, r.first, off, i, r.last);
 		const Key2 ki = load_key(key, i);
-		double sx = 0, sy = 0, sz = 0, sm = 0;
+		/* lane 4 q + c sums component c (m x, m y, m z, m): one add per
+		 * particle advances all four sums, in PH order each */
+		const int comp = lane & 3;
+		double acc = 0;
 		int cur = hi, runmin = 99;
 		for (int b0 = i; b0 <= e && cur >= lo; b0 += 64) {
 			const int k = b0 + lane;
@@ -444,16 +447,24 @@ __global__ void __launch_bounds__(64) k_tree_nodes_long(
 				const unsigned long long ends = __ballot(lane >= q0
 						&& lane < nb && dn <= cur);
 				const int qe = ends ? __ffsll((long long)ends) - 1 : nb - 1;
-				for (int q = q0; q <= qe; q++) {
-					const double4 t = sh[q];
-					sx = sx + t.x;
-					sy = sy + t.y;
-					sz = sz + t.z;
-					sm = sm + t.w;
+				const double *col = reinterpret_cast<const double *>(sh) + comp;
+				int q = q0;
+				for (; q + 8 <= qe + 1; q += 8) {
+					double t[8];
+#pragma unroll
+					for (int u = 0; u < 8; u++)
+						t[u] = col[4 * (q + u)];
+#pragma unroll
+					for (int u = 0; u < 8; u++)
+						acc = acc + t[u];
 				}
+				for (; q <= qe; q++)
+					acc = acc + col[4 * q];
 				q0 = qe + 1;
 				if (!ends)
 					break;
+				const double sx = __shfl(acc, 0, 64), sy = __shfl(acc, 1, 64),
+						sz = __shfl(acc, 2, 64), sm = __shfl(acc, 3, 64);
 				const int kn = b0 + qe + 1;
 				const int dnq = shdn[qe];
 				const int lowest = dnq > lo ? dnq : lo;
@@ -599,6 +610,7 @@ int tdv_build_tree(tdv_context *c)
 {
 	const size_t n = c->n;
 	const int vs = c->par.vector_size;
+	c->lvl_ok = 0;                       /* the FP32 walk's level lists */
 	TDV_HIP(c, hipMemsetAsync(c->minpack, 0xFF, n * sizeof(uint32_t),
 			c->stream));
 	hipLaunchKernelGGL(k_tree_marks, dim3(tdv_blocks(n, 256)), dim3(256), 0,

@@ -31,6 +31,11 @@
 #define TDV_BIT_TOP 10
 #define TDV_MAX_TOP 65536       /* top-node records a context accepts */
 
+/* stage events of a context (stage_ms: keys, sort, reorder, top nodes,
+ * build, walk, Ewald walk, kick/drift) */
+enum { EV_KEYS0, EV_KEYS1, EV_SORT1, EV_REORDER1, EV_TOP1, EV_BUILD0,
+	EV_BUILD1, EV_WALK0, EV_WALK1, EV_EWALD1, EV_KICK0, EV_KICK1, EV_N };
+
 /* error bits raised by kernels into ctx->d_err */
 #define TDV_EBIT_DUP 1u
 #define TDV_EBIT_DEPTH 2u
@@ -70,7 +75,10 @@ struct tdv_context {
 	unsigned *d_queue;               /* work-queue counter of the walk */
 	double4 *partial;                /* split walk: TDV_SPLIT_ITEMS x 64 */
 	double *q_s6;                    /* nodes' FP64 second moments, 6 each */
-	unsigned *q_arrived;             /* children counted at each node */
+	unsigned *q_arrived;             /* level counts, fills, offsets: 192 */
+	int *lvl_nodes;                  /* node indices grouped by level */
+	int lvl_off[64], lvl_cnt[64];    /* per level: first entry, count */
+	int lvl_ok;                      /* the lists match the tree */
 	hipGraph_t graph;                /* captured regular-step walk */
 	hipGraphExec_t graph_exec;
 	int capturing;

@@ -175,14 +175,13 @@ __global__ void k_prep32_offsets(size_t n, const int4 *__restrict__ p32,
  * S^2 masses, fixed lengths) as {Txx, Tyy, Txy, Txz}, {Tyz, 0, 0, 0}; Tzz =
  * -Txx - Tyy.
  *
- * Bottom-up in O(nodes + particles): every node covers the contiguous
- * particle range [first, first + Npart) and its children (the nodes from
- * j + 1 below j + DNext, siblings linked by DNext, bundles by + 1) cover
- * sub-ranges of it in PH order.  A node without child nodes takes S from
- * its particles; every node then counts itself at its parent (DUp), and the
- * child that arrives last, which sees every sibling's S, forms the parent's
- * S = sum_c (S_c + m_c d_c d_c^T), d_c = CoM_c - CoM, plus the particles no
- * child covers, and climbs on.  Nobody waits: the others leave. */
+ * Bottom-up in O(nodes + particles), one launch per tree level from the
+ * deepest: every node covers the contiguous particle range [first, first +
+ * Npart) and its children (one level deeper: the nodes from j + 1 below
+ * j + DNext, siblings linked by DNext, bundles by + 1) cover sub-ranges of
+ * it in PH order.  A node without child nodes takes S from its particles;
+ * any other node S = sum_c (S_c + m_c d_c d_c^T), d_c = CoM_c - CoM, plus
+ * the particles no child covers. */
 __device__ __forceinline__ void moments_add_particles(
 		const double4 *__restrict__ pm, int first, int count, double cx,
 		double cy, double cz, double *m6)
@@ -218,55 +217,21 @@ __device__ __forceinline__ int next_sibling(const int4 *__restrict__ nc,
 	return d > 0 ? ch + d : ch + 1;
 }
 
-__global__ void k_moments_clear(int nn, unsigned *__restrict__ arrived)
-{
-	const int j = blockIdx.x * blockDim.x + threadIdx.x;
-	if (j < nn)
-		arrived[j] = 0;
-}
-
-__global__ void __launch_bounds__(256) k_moments_up(int nn,
-		const int4 *__restrict__ nc, const int *__restrict__ up,
+__global__ void __launch_bounds__(256) k_moments_level(int count,
+		const int *__restrict__ list, const int4 *__restrict__ nc,
 		const double4 *__restrict__ na, const double4 *__restrict__ pm,
-		double *__restrict__ s6, unsigned *__restrict__ arrived, double scale,
-		float4 *__restrict__ q)
+		double *__restrict__ s6, double scale, float4 *__restrict__ q)
 {
-	int j = blockIdx.x * blockDim.x + threadIdx.x;
-	if (j >= nn)
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= count)
 		return;
-	int4 c = nc[j];
-	if (c.w < 0) {                       /* terminators: never used */
-		q[2 * j] = make_float4(0.f, 0.f, 0.f, 0.f);
-		q[2 * j + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
-		return;
-	}
-	if (c.x > 1)                         /* has child nodes: done by them */
-		return;
+	const int j = list[t];
+	const int4 c = nc[j];
+	const double4 a = na[j];
 	double m6[6] = { 0, 0, 0, 0, 0, 0 };
-	double4 a = na[j];
-	moments_add_particles(pm, c.w, c.y, a.x, a.y, a.z, m6);
-	for (;;) {
-		for (int k = 0; k < 6; k++)
-			s6[6 * (size_t)j + k] = m6[k];
-		moments_store(m6, scale, q + 2 * j);
-		if ((c.z >> TDV_BIT_TOP) & 1)
-			return;                      /* a subtree's root */
-		const int par = j - up[j];
-		const int4 pc = nc[par];
-		const int end = par + pc.x;
-		int nchild = 0;
-		for (int ch = par + 1; ch < end; ch = next_sibling(nc, ch))
-			nchild++;
-		__threadfence();                 /* this node's S before the count */
-		if (atomicAdd(&arrived[par], 1u) != (unsigned)(nchild - 1))
-			return;
-		__threadfence();                 /* every sibling's S after it */
-		j = par;
-		c = pc;
-		a = na[j];
-		for (int k = 0; k < 6; k++)
-			m6[k] = 0;
+	if (c.x > 1) {
 		int covered = c.w;
+		const int end = j + c.x;
 		for (int ch = j + 1; ch < end; ch = next_sibling(nc, ch)) {
 			const int4 cc = nc[ch];
 			moments_add_particles(pm, covered, cc.w - covered, a.x, a.y, a.z,
@@ -285,7 +250,60 @@ __global__ void __launch_bounds__(256) k_moments_up(int nn,
 		}
 		moments_add_particles(pm, covered, c.w + c.y - covered, a.x, a.y,
 				a.z, m6);
+	} else {
+		moments_add_particles(pm, c.w, c.y, a.x, a.y, a.z, m6);
 	}
+	for (int k = 0; k < 6; k++)
+		s6[6 * (size_t)j + k] = m6[k];
+	moments_store(m6, scale, q + 2 * j);
+}
+
+/* per-level node lists (terminators left out); counts are gathered per
+ * block in LDS first, so the device-wide counters take one atomic per level
+ * and block */
+__global__ void __launch_bounds__(256) k_level_count(int nn,
+		const int4 *__restrict__ nc, unsigned *__restrict__ cnt)
+{
+	__shared__ unsigned h[64];
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (threadIdx.x < 64)
+		h[threadIdx.x] = 0;
+	__syncthreads();
+	if (j < nn) {
+		const int4 c = nc[j];
+		if (c.w >= 0)
+			atomicAdd(&h[c.z & 63], 1u);
+	}
+	__syncthreads();
+	if (threadIdx.x < 64 && h[threadIdx.x])
+		atomicAdd(&cnt[threadIdx.x], h[threadIdx.x]);
+}
+
+__global__ void __launch_bounds__(256) k_level_fill(int nn,
+		const int4 *__restrict__ nc, const int *__restrict__ off,
+		unsigned *__restrict__ fill, int *__restrict__ list)
+{
+	__shared__ unsigned h[64], base[64];
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (threadIdx.x < 64)
+		h[threadIdx.x] = 0;
+	__syncthreads();
+	int lv = -1;
+	unsigned r = 0;
+	if (j < nn) {
+		const int4 c = nc[j];
+		if (c.w >= 0) {
+			lv = c.z & 63;
+			r = atomicAdd(&h[lv], 1u);
+		}
+	}
+	__syncthreads();
+	if (threadIdx.x < 64)
+		base[threadIdx.x] = h[threadIdx.x]
+				? atomicAdd(&fill[threadIdx.x], h[threadIdx.x]) : 0u;
+	__syncthreads();
+	if (lv >= 0)
+		list[off[lv] + (int)(base[lv] + r)] = j;
 }
 
 /* top nodes: from the subtree's root, shifted to the float centre of mass
@@ -941,6 +959,13 @@ __global__ void k_walk32_parts_sum(const int *__restrict__ nact_dev, int split,
 	cost[ip] = a.w;
 }
 
+__global__ void k_fill_u32(int n, unsigned *__restrict__ p)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		p[i] = 0;
+}
+
 __global__ void k_walk32_queue_reset(unsigned *__restrict__ queue,
 		unsigned long long *__restrict__ counter)
 {
@@ -995,10 +1020,11 @@ int tdv_walk32(tdv_context *c, int use_bh)
 			(void)hipFree(c->q32);
 		if (c->q_s6)
 			(void)hipFree(c->q_s6);
-		if (c->q_arrived)
-			(void)hipFree(c->q_arrived);
+		if (c->lvl_nodes)
+			(void)hipFree(c->lvl_nodes);
 		c->q_s6 = nullptr;
-		c->q_arrived = nullptr;
+		c->lvl_nodes = nullptr;
+		c->lvl_ok = 0;
 		c->a32 = c->b32 = nullptr;
 		c->q32 = nullptr;
 		c->n32_cap = 0;
@@ -1008,8 +1034,8 @@ int tdv_walk32(tdv_context *c, int use_bh)
 				2 * c->node_cap * sizeof(float4)));
 		TDV_HIP(c, hipMalloc((void **)&c->q_s6,
 				6 * c->node_cap * sizeof(double)));
-		TDV_HIP(c, hipMalloc((void **)&c->q_arrived,
-				c->node_cap * sizeof(unsigned)));
+		TDV_HIP(c, hipMalloc((void **)&c->lvl_nodes,
+				c->node_cap * sizeof(int)));
 		c->n32_cap = c->node_cap;
 	}
 	const bool quad = c->par.fp32_order != 1;
@@ -1032,13 +1058,42 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	/* quadrupoles: masses G S^2, lengths S -> scale G S^4 */
 	const double qscale = mscale * S * S;
 	if (quad) {
-		if (nn) {
-			hipLaunchKernelGGL(k_moments_clear, dim3(tdv_blocks(nn, 256)),
-					dim3(256), 0, c->stream, (int)nn, c->q_arrived);
-			hipLaunchKernelGGL(k_moments_up, dim3(tdv_blocks(nn, 256)),
-					dim3(256), 0, c->stream, (int)nn, c->node_c, c->node_up,
-					c->node_a, c->pm, c->q_s6, c->q_arrived, qscale, c->q32);
+		if (nn && !c->lvl_ok) {
+			/* the tree's per-level node lists, once per build (a host sync;
+			 * a captured walk requires them made) */
+			if (c->capturing)
+				return tdv_fail(c, TDV_ERR_ARG, "capture: levels missing");
+			unsigned *cnt = c->q_arrived, *fill = c->q_arrived + 64;
+			int *off = reinterpret_cast<int *>(c->q_arrived + 128);
+			hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(128), 0, c->stream,
+					128, c->q_arrived);
+			hipLaunchKernelGGL(k_level_count, dim3(tdv_blocks(nn, 256)),
+					dim3(256), 0, c->stream, (int)nn, c->node_c, cnt);
+			unsigned h[64];
+			TDV_HIP(c, hipMemcpyAsync(h, cnt, sizeof(h),
+					hipMemcpyDeviceToHost, c->stream));
+			TDV_HIP(c, hipStreamSynchronize(c->stream));
+			int run = 0;
+			for (int lv = 0; lv < 64; lv++) {
+				c->lvl_off[lv] = run;
+				c->lvl_cnt[lv] = (int)h[lv];
+				run += (int)h[lv];
+			}
+			TDV_HIP(c, hipMemcpyAsync(off, c->lvl_off, 64 * sizeof(int),
+					hipMemcpyHostToDevice, c->stream));
+			hipLaunchKernelGGL(k_level_fill, dim3(tdv_blocks(nn, 256)),
+					dim3(256), 0, c->stream, (int)nn, c->node_c, off, fill,
+					c->lvl_nodes);
+			TDV_HIP(c, hipStreamSynchronize(c->stream));   /* c->lvl_off */
+			c->lvl_ok = 1;
 		}
+		for (int lv = 63; nn && lv >= 0; lv--)
+			if (c->lvl_cnt[lv] > 0)
+				hipLaunchKernelGGL(k_moments_level,
+						dim3(tdv_blocks((size_t)c->lvl_cnt[lv], 256)),
+						dim3(256), 0, c->stream, c->lvl_cnt[lv],
+						c->lvl_nodes + c->lvl_off[lv], c->node_c, c->node_a,
+						c->pm, c->q_s6, qscale, c->q32);
 		hipLaunchKernelGGL(k_moments_top, dim3(tdv_blocks((size_t)c->ntop,
 				64)), dim3(64), 0, c->stream, c->ntop, c->top,
 				c->par.vector_size, c->node_a, c->pm, c->q_s6, qscale, c->tq32);

@@ -107,6 +107,7 @@ def load_library(path=LIB_PATH):
     L.tdv_set_tree_moments.argtypes = [_ctx, _vp, _sz, _vp, ctypes.c_int]
     L.tdv_get_ewald_derivs.argtypes = [_ctx, _dp]
     L.tdv_get_walk32_quadrupoles.argtypes = [_ctx, _vp, _vp, _vp]
+    L.tdv_kick_drift.argtypes = [_ctx, ctypes.c_double, ctypes.c_double]
     L.tdv_capture_walk.argtypes = [_ctx]
     L.tdv_launch_walk.argtypes = [_ctx]
     L.tdv_num_particles.argtypes = [_ctx]
@@ -446,6 +447,12 @@ class Engine:
                                        n.value, ctypes.byref(n)),
                  "tdv_get_tree")
         return buf
+
+    def kick_drift(self, kick, drift):
+        """Kick the active particles (v += kick a), then drift every particle
+        (x += drift v, wrapped into the box when periodic), on the device."""
+        self._ok(self.lib.tdv_kick_drift(self._c, float(kick), float(drift)),
+                 "tdv_kick_drift")
 
     def walk32_quadrupoles(self):
         """The FP32 walk's quadrupole records of the last FP32 walk: (nodes

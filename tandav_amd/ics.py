@@ -68,18 +68,24 @@ def _zeldovich_displacement(ngrid, seed, slope, rms_frac):
     return psi * (rms_frac / ngrid / rms)
 
 
-def zeldovich_box(ngrid=128, seed=7, slope=-2.5, rms_frac=0.1, box=1.0):
-    """Regular grid + Zel'dovich displacement, wrapped into [0, box)."""
+def zeldovich_box(ngrid=128, seed=7, slope=-2.5, rms_frac=0.1, box=1.0,
+                  vel_factor=0.0):
+    """Regular grid + Zel'dovich displacement psi, wrapped into [0, box);
+    velocities vel_factor * psi (the growing mode: a H f for comoving
+    velocities), zero by default."""
     psi = _zeldovich_displacement(ngrid, seed, slope, rms_frac)
-    i = np.arange(ngrid, dtype=np.float64)
-    gx, gy, gz = np.meshgrid(i, i, i, indexing="ij")
-    q = (np.stack([gx.reshape(-1), gy.reshape(-1), gz.reshape(-1)], 1)
-         + 0.5) / ngrid
-    pos = (q + psi) * box
-    pos = np.mod(pos, box)
-    pos[pos >= box] -= box          # np.mod can round up to box
     n = ngrid ** 3
-    return pos, np.zeros((n, 3)), np.full(n, 1.0 / n)
+    vel = psi * (vel_factor * box) if vel_factor else np.zeros((n, 3))
+    i = (np.arange(ngrid, dtype=np.float64) + 0.5) / ngrid
+    pos = psi
+    pos[:, 0] += np.repeat(i, ngrid * ngrid)
+    pos[:, 1] += np.tile(np.repeat(i, ngrid), ngrid)
+    pos[:, 2] += np.tile(i, ngrid * ngrid)
+    if box != 1.0:
+        pos *= box
+    pos = np.mod(pos, box, out=pos)
+    pos[pos >= box] -= box          # np.mod can round up to box
+    return pos, vel, np.full(n, 1.0 / n)
 
 
 def clustered_box(n=1 << 23, nhalo=256, seed=11, background=0.2,
