@@ -203,13 +203,15 @@ int tdv_ewald_derivs_device(tdv_context *c, double *out10)
  * same pairwise Newton kernel as k_direct (nearest image, K1 softening),
  * plus -F(d / Box) / Box^2 from the Ewald sums themselves (ewald_F_any)
  * instead of the 65^3 table's trilinear interpolation.  An error oracle for
- * the walks that is free of the table's interpolation error; one workgroup
- * per target, lanes over the sources.
+ * the walks that is free of the table's interpolation error.  Every pair
+ * costs a full Ewald sum, so the sources of each target are cut into
+ * chunks (blockIdx.y) to fill the chip whatever the number of targets;
+ * k_exact_chunks_sum adds the chunks' sums in order.
  */
 __global__ void __launch_bounds__(256) k_direct_exact(int nt,
 		const int *__restrict__ idx, const double4 *__restrict__ pm, int n,
-		double box, double G, double eps2, double eps3,
-		double *__restrict__ out)
+		int chunk, double box, double G, double eps2, double eps3,
+		double *__restrict__ part)
 {
 	__shared__ double red[3][256];
 	const int t = blockIdx.x;
@@ -218,8 +220,11 @@ __global__ void __launch_bounds__(256) k_direct_exact(int nt,
 	const int i = idx[t];
 	const double4 p = pm[i];
 	const double half = box / 2.0, ib = 1.0 / box;
+	const long long j0 = (long long)blockIdx.y * chunk;
+	const long long j1 = j0 + chunk < n ? j0 + chunk : n;
 	double ax = 0, ay = 0, az = 0;
-	for (int j = threadIdx.x; j < n; j += blockDim.x) {
+	for (long long jj = j0 + threadIdx.x; jj < j1; jj += blockDim.x) {
+		const int j = (int)jj;
 		if (j == i)
 			continue;
 		const double4 q = pm[j];
@@ -261,7 +266,21 @@ __global__ void __launch_bounds__(256) k_direct_exact(int nt,
 	}
 	if (threadIdx.x == 0)
 		for (int d = 0; d < 3; d++)
-			out[3 * t + d] = red[d][0];
+			part[((size_t)t * gridDim.y + blockIdx.y) * 3 + d] = red[d][0];
+}
+
+__global__ void k_exact_chunks_sum(int nt, int nchunk,
+		const double *__restrict__ part, double *__restrict__ out)
+{
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= nt)
+		return;
+	double a[3] = { 0, 0, 0 };
+	for (int ch = 0; ch < nchunk; ch++)
+		for (int d = 0; d < 3; d++)
+			a[d] += part[((size_t)t * nchunk + ch) * 3 + d];
+	for (int d = 0; d < 3; d++)
+		out[3 * t + d] = a[d];
 }
 
 int tdv_direct_sum_exact(tdv_context *c, int nt, const int *idx_dev,
@@ -271,8 +290,26 @@ int tdv_direct_sum_exact(tdv_context *c, int nt, const int *idx_dev,
 		return 0;
 	if (!c->par.periodic)
 		return tdv_direct_sum(c, nt, idx_dev, out_dev);
-	hipLaunchKernelGGL(k_direct_exact, dim3(nt), dim3(256), 0, c->stream, nt,
-			idx_dev, c->pm, (int)c->n, c->par.boxsize, c->par.gravity,
-			c->Epsilon2, c->Epsilon3, out_dev);
-	return tdv_check(c, hipGetLastError(), "k_direct_exact");
+	/* about 4096 workgroups in all, chunks of at least 4096 sources */
+	const size_t n = c->n;
+	size_t nchunk = (4096 + (size_t)nt - 1) / (size_t)nt;
+	if (nchunk > (n + 4095) / 4096)
+		nchunk = (n + 4095) / 4096;
+	if (nchunk < 1)
+		nchunk = 1;
+	if (nchunk > 65535)
+		nchunk = 65535;
+	const size_t chunk = (n + nchunk - 1) / nchunk;
+	double *part = nullptr;
+	TDV_HIP(c, hipMalloc((void **)&part, (size_t)nt * nchunk * 3 * sizeof(double)));
+	hipLaunchKernelGGL(k_direct_exact, dim3(nt, (unsigned)nchunk), dim3(256), 0,
+			c->stream, nt, idx_dev, c->pm, (int)n, (int)chunk, c->par.boxsize,
+			c->par.gravity, c->Epsilon2, c->Epsilon3, part);
+	hipLaunchKernelGGL(k_exact_chunks_sum, dim3(tdv_blocks((size_t)nt, 256)),
+			dim3(256), 0, c->stream, nt, (int)nchunk, part, out_dev);
+	const int r = tdv_check(c, hipGetLastError(), "k_direct_exact");
+	const int rs = tdv_check(c, hipStreamSynchronize(c->stream),
+			"k_direct_exact");
+	(void)hipFree(part);
+	return r ? r : rs;
 }
