@@ -751,16 +751,35 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 					 * correction taken there: d = (A - me) + (q - A) */
 					const float cx = fdiff(A.x, me.x), cy = fdiff(A.y, me.y),
 							cz = fdiff(A.z, me.z);
-					/* four particles per wide scalar load */
+					/* four particles per wide scalar load, the next four
+					 * loaded while these are taken (the load's latency
+					 * hides behind the arithmetic) */
 					int q = f;
-					for (; q + 4 <= l; q += 4) {
-						const float4 p0 = p32o[q], p1 = p32o[q + 1],
-								p2 = p32o[q + 2], p3 = p32o[q + 3];
-						newton32(a, p0.w, cx + p0.x, cy + p0.y, cz + p0.z, w);
-						newton32(a, p1.w, cx + p1.x, cy + p1.y, cz + p1.z, w);
-						newton32(a, p2.w, cx + p2.x, cy + p2.y, cz + p2.z, w);
-						newton32(a, p3.w, cx + p3.x, cy + p3.y, cz + p3.z, w);
-						tickn(a, 4, every);
+					if (q + 4 <= l) {
+						float4 n0 = p32o[q], n1 = p32o[q + 1],
+								n2 = p32o[q + 2], n3 = p32o[q + 3];
+						for (;;) {
+							const float4 p0 = n0, p1 = n1, p2 = n2, p3 = n3;
+							const bool more = q + 8 <= l;
+							if (more) {
+								n0 = p32o[q + 4];
+								n1 = p32o[q + 5];
+								n2 = p32o[q + 6];
+								n3 = p32o[q + 7];
+							}
+							newton32(a, p0.w, cx + p0.x, cy + p0.y, cz + p0.z,
+									w);
+							newton32(a, p1.w, cx + p1.x, cy + p1.y, cz + p1.z,
+									w);
+							newton32(a, p2.w, cx + p2.x, cy + p2.y, cz + p2.z,
+									w);
+							newton32(a, p3.w, cx + p3.x, cy + p3.y, cz + p3.z,
+									w);
+							tickn(a, 4, every);
+							q += 4;
+							if (!more)
+								break;
+						}
 					}
 					for (; q < l; q++) {
 						const float4 p0 = p32o[q];
