@@ -267,6 +267,12 @@ int tdv_set_active(tdv_context *ctx, int nactive, const int *list);
 int tdv_set_tree_moments(tdv_context *ctx, const void *tree, size_t nnodes,
 		const void *top, int ntop);
 
+/* The FP32 walk's launch options of an existing context (the tdv_params
+ * fields fp32_waves, fp32_queue, fp32_split, same ranges): they change the
+ * schedule, not the interactions, so an autotuner can time them on one tree.
+ * Drops a captured walk.  No counterpart in the reference. */
+int tdv_set_fp32_options(tdv_context *ctx, int waves, int queue, int split);
+
 /* Kick_First/Second_Halfstep (kick.c:9-70) for the active particles,
  * v += kick * Acc, then Drift_To_Sync_Point (drift.c:12-64) for every
  * particle, x += drift * v, wrapped into the box on periodic runs

@@ -1191,6 +1191,19 @@ int tdv_interaction_counts(tdv_context *c, uint64_t counts[2])
 	return 0;
 }
 
+int tdv_set_fp32_options(tdv_context *c, int waves, int queue, int split)
+{
+	if (!c || (waves != 0 && waves != 6 && waves != 8)
+			|| (queue != 0 && queue != 1) || split < 0 || split > 256)
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_set_fp32_options: waves 0, 6 "
+				"or 8, queue 0 or 1, split 0..256");
+	c->par.fp32_waves = waves;
+	c->par.fp32_queue = queue;
+	c->par.fp32_split = split;
+	tdv_graph_drop(c);
+	return 0;
+}
+
 int tdv_stage_times(tdv_context *c, float ms[8])
 {
 	if (!c || !ms)

@@ -108,6 +108,8 @@ def load_library(path=LIB_PATH):
     L.tdv_get_ewald_derivs.argtypes = [_ctx, _dp]
     L.tdv_get_walk32_quadrupoles.argtypes = [_ctx, _vp, _vp, _vp]
     L.tdv_kick_drift.argtypes = [_ctx, ctypes.c_double, ctypes.c_double]
+    L.tdv_set_fp32_options.argtypes = [_ctx, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int]
     L.tdv_capture_walk.argtypes = [_ctx]
     L.tdv_launch_walk.argtypes = [_ctx]
     L.tdv_num_particles.argtypes = [_ctx]
@@ -447,6 +449,17 @@ class Engine:
                                        n.value, ctypes.byref(n)),
                  "tdv_get_tree")
         return buf
+
+    def set_fp32_options(self, waves=None, queue=None, split=None):
+        """Change the FP32 walk's launch options (register budget, work-queue
+        kernel, parts per group); None keeps the current value."""
+        p = self.params
+        w = p.fp32_waves if waves is None else int(waves)
+        q = p.fp32_queue if queue is None else int(queue)
+        s = p.fp32_split if split is None else int(split)
+        self._ok(self.lib.tdv_set_fp32_options(self._c, w, q, s),
+                 "tdv_set_fp32_options")
+        p.fp32_waves, p.fp32_queue, p.fp32_split = w, q, s
 
     def kick_drift(self, kick, drift):
         """Kick the active particles (v += kick a), then drift every particle
