@@ -202,6 +202,13 @@ int tdv_set_ewald_table(tdv_context *ctx, const double *table4);
  * second order; the reference has no counterpart. */
 int tdv_get_ewald_derivs(tdv_context *ctx, double *s10);
 
+/* First derivatives J_ab = d_b E_a of the same correction at the same nodes,
+ * per unit box: 6 x 65^3 doubles, components xx yy zz xy xz yz (E is a
+ * gradient, so J is symmetric).  With the second derivatives they give the
+ * FP32 walk the correction near each node to second order; the reference
+ * has no counterpart. */
+int tdv_get_ewald_jacobian(tdv_context *ctx, double *j6);
+
 /* The FP32 walk's quadrupole records of the last FP32 walk: per tree node
  * (tdv_get_tree's order) and per top node 8 floats, {Txx, Tyy, Txy, Txz,
  * Tyz, 0, 0, 0}, T = sum m (3 y y^T - |y|^2 I) about the node's centre of

@@ -96,6 +96,19 @@ void tdv_graph_drop(tdv_context *c)
 	c->graph = nullptr;
 }
 
+/* the first and second derivatives of the Ewald correction at the table
+ * nodes, computed once per context */
+int tdv_ewald_derivs_ready(tdv_context *c)
+{
+	if (c->ewald_s && c->ewald_j)
+		return 0;
+	if (!c->ewald_s)
+		TDV_TRY(dalloc(c, &c->ewald_s, 10 * (size_t)TDV_NE3));
+	if (!c->ewald_j)
+		TDV_TRY(dalloc(c, &c->ewald_j, 6 * (size_t)TDV_NE3));
+	return tdv_ewald_derivs_device(c, c->ewald_s, c->ewald_j);
+}
+
 static int set_nactive(tdv_context *c, int nact)
 {
 	c->nactive = nact;
@@ -339,6 +352,7 @@ void tdv_destroy(tdv_context *c)
 	dfree(c->ewald);
 	dfree(c->ewald16);
 	dfree(c->ewald_s);
+	dfree(c->ewald_j);
 	dfree(c->p32);
 	dfree(c->p32o);
 	dfree(c->a32);
@@ -927,13 +941,20 @@ int tdv_get_ewald_derivs(tdv_context *c, double *s10)
 {
 	if (!c || !s10 || !c->par.periodic)
 		return tdv_fail(c, TDV_ERR_ARG, "no Ewald derivatives on this run");
-	if (!c->ewald_s) {
-		TDV_TRY(dalloc(c, &c->ewald_s, 10 * (size_t)TDV_NE3));
-		TDV_TRY(tdv_ewald_derivs_device(c, c->ewald_s));
-	}
+	TDV_TRY(tdv_ewald_derivs_ready(c));
 	TDV_HIP(c, hipMemcpyAsync(s10, c->ewald_s, 10 * (size_t)TDV_NE3 * 8,
 			hipMemcpyDeviceToHost, c->stream));
 	return tdv_check(c, hipStreamSynchronize(c->stream), "ewald derivs");
+}
+
+int tdv_get_ewald_jacobian(tdv_context *c, double *j6)
+{
+	if (!c || !j6 || !c->par.periodic)
+		return tdv_fail(c, TDV_ERR_ARG, "no Ewald derivatives on this run");
+	TDV_TRY(tdv_ewald_derivs_ready(c));
+	TDV_HIP(c, hipMemcpyAsync(j6, c->ewald_j, 6 * (size_t)TDV_NE3 * 8,
+			hipMemcpyDeviceToHost, c->stream));
+	return tdv_check(c, hipStreamSynchronize(c->stream), "ewald jacobian");
 }
 
 int tdv_get_walk32_quadrupoles(tdv_context *c, float *nodes, float *top,

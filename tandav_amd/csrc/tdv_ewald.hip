@@ -150,7 +150,8 @@ __device__ void ewald_F_any(const double x[3], double f[3])
 			}
 }
 
-__global__ void __launch_bounds__(64) k_ewald_derivs(double *__restrict__ s)
+__global__ void __launch_bounds__(64) k_ewald_derivs(double *__restrict__ s,
+		double *__restrict__ jac)
 {
 	const int n = blockIdx.x * blockDim.x + threadIdx.x;
 	if (n >= TDV_NE3)
@@ -189,12 +190,32 @@ __global__ void __launch_bounds__(64) k_ewald_derivs(double *__restrict__ s)
 		-d2[0][2], -d2[1][0], -d2[1][2], -d2[2][0], -d2[2][1], -dyz_x };
 	for (int q = 0; q < 10; q++)
 		s[(size_t)q * TDV_NE3 + n] = v[q];
+	if (jac) {
+		/* J_ab = d_b E_a from the same central differences, symmetrised
+		 * (E is a gradient): xx yy zz xy xz yz */
+		double pj[3][3];                  /* pj[axis][component] = d F */
+		for (int ax = 0; ax < 3; ax++) {
+			double xp[3] = { x0[0], x0[1], x0[2] },
+					xm[3] = { x0[0], x0[1], x0[2] };
+			xp[ax] += H;
+			xm[ax] -= H;
+			ewald_F_any(xp, p);
+			ewald_F_any(xm, m);
+			for (int q = 0; q < 3; q++)
+				pj[ax][q] = (p[q] - m[q]) / (2.0 * H);
+		}
+		const double jv[6] = { -pj[0][0], -pj[1][1], -pj[2][2],
+			-0.5 * (pj[1][0] + pj[0][1]), -0.5 * (pj[2][0] + pj[0][2]),
+			-0.5 * (pj[2][1] + pj[1][2]) };
+		for (int q = 0; q < 6; q++)
+			jac[(size_t)q * TDV_NE3 + n] = jv[q];
+	}
 }
 
-int tdv_ewald_derivs_device(tdv_context *c, double *out10)
+int tdv_ewald_derivs_device(tdv_context *c, double *out10, double *out6)
 {
 	hipLaunchKernelGGL(k_ewald_derivs, dim3(tdv_blocks(TDV_NE3, 64)), dim3(64),
-			0, c->stream, out10);
+			0, c->stream, out10, out6);
 	return tdv_check(c, hipGetLastError(), "k_ewald_derivs");
 }
 

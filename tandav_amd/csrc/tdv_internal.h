@@ -122,8 +122,10 @@ struct tdv_context {
 	float4 *tq32;                    /* 2 per top node */
 	double *ewald_s;                 /* 10 * TDV_NE3: second derivatives
 	                                    of the correction, per unit box */
+	double *ewald_j;                 /* 6 * TDV_NE3: first derivatives
+	                                    d_b E_a, xx yy zz xy xz yz */
 	float4 *ewald16;                 /* FP32 walk: 4 float4 per grid node,
-	                                    {E, S} in kernel units */
+	                                    {E, J, S} in kernel units */
 	int ewald16_ok;                  /* ewald16 matches ewald */
 
 	unsigned *d_err;
@@ -180,7 +182,8 @@ int tdv_direct_sum_exact(tdv_context *c, int nt, const int *idx_dev,
 		double *out_dev);
 void tdv_launch_safe_last_accel(tdv_context *c);
 int tdv_ewald_table_device(tdv_context *c, double *out4_unnormalised);
-int tdv_ewald_derivs_device(tdv_context *c, double *out10);
+int tdv_ewald_derivs_device(tdv_context *c, double *out10, double *out6);
+int tdv_ewald_derivs_ready(tdv_context *c);
 void tdv_launch_ewald_correction(tdv_context *c, size_t n, const double *dr,
 		double *f);
 

@@ -44,7 +44,8 @@ struct Walk32Params {
 	float u_scale;                   /* fixed units -> table grid units */
 	float inv_box_f;                 /* 1 / box_f */
 	float inv_s2;                    /* 1 / S^2: quadrupoles -> table units */
-	float tri_corr;                  /* trilinear error from S, table units */
+	float taylor2;                   /* S/6 per unit box -> (1/2) S per grid
+	                                    unit in E's units */
 	float inv_s2_box4;               /* 1 / (S^2 Box^4) */
 	int carry_every;
 	int quad;                        /* accepted nodes add their quadrupole */
@@ -109,13 +110,15 @@ __global__ void k_prep32_top(int ntop, const TopNodes T, double o0, double o1,
 			__float_as_int((float)m));
 }
 
-/* The walk's Ewald records, one per table node, 32 bytes: {Ex, Ey, Ez,
- * (Sxxx, Syyy)}, {(Szzz, Sxxy), (Sxxz, Sxyy), (Syyz, Sxzz), (Syzz, Sxyz)},
- * pairs of FP16 in one 32-bit word.  E = -F (the table's sign rule:
- * Ewald_Correction returns -F(|d|) on the positive octant) in units where
- * m' E is an acceleration (m' = G m S^2).  S = d d E / 6 per unit box: the
- * second derivatives only enter second-order terms, where FP16's 5e-4
- * relative precision is far below the terms' own size. */
+/* The walk's Ewald records, one per table node, 64 bytes: E, its first
+ * derivatives J (FP32) and its second derivatives S / 6 (FP16 pairs):
+ *   {Ex, Ey, Ez, Jxx}, {Jyy, Jzz, Jxy, Jxz},
+ *   {Jyz, (Sxxx, Syyy), (Szzz, Sxxy), (Sxxz, Sxyy)}, {(Syyz, Sxzz), (Syzz, Sxyz), 0, 0}.
+ * E = -F (the table's sign rule: Ewald_Correction returns -F(|d|) on the
+ * positive octant) in units where m' E is an acceleration (m' = G m S^2);
+ * J per table grid unit in the same units; S per unit box (it enters the
+ * node's quadrupole term, and, rescaled, the expansion's second-order term,
+ * where FP16's 5e-4 relative precision is far below the terms' own size). */
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ float pack_h2(double a, double b)
@@ -126,22 +129,27 @@ __device__ __forceinline__ float pack_h2(double a, double b)
 	return __builtin_bit_cast(float, h);
 }
 
-__global__ void k_prep32_ewald16(const double *__restrict__ raw,
-		const double *__restrict__ s10, double e_scale,
-		float4 *__restrict__ t16)
+__global__ void k_prep32_ewald_taylor(const double *__restrict__ raw,
+		const double *__restrict__ s10, const double *__restrict__ j6,
+		double e_scale, double j_scale, float4 *__restrict__ t)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= TDV_NE3)
 		return;
-	double S[10];
+	double S[10], J[6];
 	for (int q = 0; q < 10; q++)
 		S[q] = s10[(size_t)q * TDV_NE3 + i] / 6.0;
-	t16[2 * i] = make_float4((float)(-raw[i] * e_scale),
+	for (int q = 0; q < 6; q++)
+		J[q] = j6[(size_t)q * TDV_NE3 + i] * j_scale;
+	t[4 * i] = make_float4((float)(-raw[i] * e_scale),
 			(float)(-raw[(size_t)TDV_NE3 + i] * e_scale),
-			(float)(-raw[2 * (size_t)TDV_NE3 + i] * e_scale),
-			pack_h2(S[0], S[1]));
-	t16[2 * i + 1] = make_float4(pack_h2(S[2], S[3]), pack_h2(S[4], S[5]),
-			pack_h2(S[6], S[7]), pack_h2(S[8], S[9]));
+			(float)(-raw[2 * (size_t)TDV_NE3 + i] * e_scale), (float)J[0]);
+	t[4 * i + 1] = make_float4((float)J[1], (float)J[2], (float)J[3],
+			(float)J[4]);
+	t[4 * i + 2] = make_float4((float)J[5], pack_h2(S[0], S[1]),
+			pack_h2(S[2], S[3]), pack_h2(S[4], S[5]));
+	t[4 * i + 3] = make_float4(pack_h2(S[6], S[7]), pack_h2(S[8], S[9]), 0.f,
+			0.f);
 }
 
 /* Every particle of a bundle: its offset from the bundle's centre of mass
@@ -415,64 +423,34 @@ __device__ __forceinline__ void quad32(Acc32 &a, const float4 qa, float qyz,
 	a.z = fmaf(ri4, fmaf(c, uz, -tuz), a.z);
 }
 
-/* Trilinear corners of the Ewald table for a separation d (fixed units):
- * the positive-octant cell, fractions, and the reflection signs (the
- * correction is odd in each axis, sign(0) = +1; Gravity/periodic.c:35-113). */
-struct EwaldCell {
-	int b;                           /* corner (i, j, k) */
-	float fu, fv, fq;
+/* The Ewald table node nearest to a separation d (fixed units) on the
+ * positive octant, the offset from it in grid units (|offset| <= 1/2 per
+ * axis) and the reflection signs (the correction is odd in each axis,
+ * sign(0) = +1; Gravity/periodic.c:35-113). */
+struct EwaldNode {
+	int b;                           /* node (i, j, k) */
+	float du, dv, dq;
 	float sx, sy, sz;
 };
 
-__device__ __forceinline__ EwaldCell ewald_cell(float dx, float dy, float dz,
+__device__ __forceinline__ EwaldNode ewald_node(float dx, float dy, float dz,
 		const Walk32Params &w)
 {
-	EwaldCell c;
+	EwaldNode c;
 	c.sx = dx < 0.f ? -1.f : 1.f;
 	c.sy = dy < 0.f ? -1.f : 1.f;
 	c.sz = dz < 0.f ? -1.f : 1.f;
-	float u = fabsf(dx) * w.u_scale, v = fabsf(dy) * w.u_scale,
+	const float u = fabsf(dx) * w.u_scale, v = fabsf(dy) * w.u_scale,
 			q = fabsf(dz) * w.u_scale;
-	const int i = min((int)u, TDV_NE - 1), j = min((int)v, TDV_NE - 1),
-			k = min((int)q, TDV_NE - 1);
-	c.fu = u - (float)i;
-	c.fv = v - (float)j;
-	c.fq = q - (float)k;
+	const int i = min((int)(u + 0.5f), TDV_NE), j = min((int)(v + 0.5f), TDV_NE),
+			k = min((int)(q + 0.5f), TDV_NE);
+	c.du = u - (float)i;
+	c.dv = v - (float)j;
+	c.dq = q - (float)k;
 	c.b = (i * TDV_NE1 + j) * TDV_NE1 + k;
 	return c;
 }
 
-/* Ewald_Correction (Gravity/periodic.c:35-113) of a point mass m' */
-__device__ __forceinline__ void ewald16(Acc32 &a, const float4 *__restrict__ t,
-		float m, float dx, float dy, float dz, const Walk32Params &w)
-{
-	const EwaldCell c = ewald_cell(dx, dy, dz, w);
-	float ex = 0.f, ey = 0.f, ez = 0.f;
-#pragma unroll
-	for (int n = 0; n < 8; n++) {
-		const int di = n >> 2, dj = (n >> 1) & 1, dk = n & 1;
-		const float wt = (di ? c.fu : 1.f - c.fu) * (dj ? c.fv : 1.f - c.fv)
-				* (dk ? c.fq : 1.f - c.fq);
-		const float4 r0 = t[2 * (c.b + (di * TDV_NE1 + dj) * TDV_NE1 + dk)];
-		ex = fmaf(wt, r0.x, ex);
-		ey = fmaf(wt, r0.y, ey);
-		ez = fmaf(wt, r0.z, ez);
-	}
-	a.x = fmaf(m * c.sx, ex, a.x);
-	a.y = fmaf(m * c.sy, ey, a.y);
-	a.z = fmaf(m * c.sz, ez, a.z);
-	a.ewald++;
-}
-
-/* The correction of a node of mass m' and traceless second moment T'
- * {Txx, Tyy, Txy, Txz}, Tyz about its centre of mass at d, to second order:
- * m' E(d) + Q:S(d) with Q:S = T:S/6 (S = d d E is symmetric in its three
- * indices and traceless in the last two, so the trace part of the second
- * moment adds nothing), and the trilinear interpolation's own second-order
- * error, u(1-u) h^2 d_k^2 E / 2 per axis, taken off with the same S.  The
- * reflection signs (S_abc is odd in each axis that appears an odd number of
- * times) and the trilinear correction are folded into seven coefficients,
- * so each corner adds three 7-term dot products. */
 __device__ __forceinline__ float h2lo(float v)
 {
 	return (float)__builtin_bit_cast(half2_t, v).x;
@@ -483,48 +461,94 @@ __device__ __forceinline__ float h2hi(float v)
 	return (float)__builtin_bit_cast(half2_t, v).y;
 }
 
+/* The correction on the positive octant from the nearest node to second
+ * order, E + J d + (1/2) S d d (d the offset in grid units; S fully
+ * symmetric, the gradient's second derivatives), and the node's S / 6. */
+struct EwaldLocal {
+	float ex, ey, ez;
+	float Sxxx, Syyy, Szzz, Sxxy, Sxxz, Sxyy, Syyz, Sxzz, Syzz, Sxyz;
+};
+
+__device__ __forceinline__ EwaldLocal ewald_local(const float4 *__restrict__ t,
+		const EwaldNode &c, const Walk32Params &w)
+{
+	const float4 *r = t + 4 * c.b;
+	const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+	EwaldLocal e;
+	e.Sxxx = h2lo(r2.y);
+	e.Syyy = h2hi(r2.y);
+	e.Szzz = h2lo(r2.z);
+	e.Sxxy = h2hi(r2.z);
+	e.Sxxz = h2lo(r2.w);
+	e.Sxyy = h2hi(r2.w);
+	e.Syyz = h2lo(r3.x);
+	e.Sxzz = h2hi(r3.x);
+	e.Syzz = h2lo(r3.y);
+	e.Sxyz = h2hi(r3.y);
+	const float Jxx = r0.w, Jyy = r1.x, Jzz = r1.y, Jxy = r1.z, Jxz = r1.w,
+			Jyz = r2.x;
+	const float uu = c.du * c.du, vv = c.dv * c.dv, qq = c.dq * c.dq;
+	const float uv = 2.f * c.du * c.dv, uq = 2.f * c.du * c.dq,
+			vq = 2.f * c.dv * c.dq;
+	/* w.taylor2 = (1/2) 6 S/6 per unit box -> per grid unit, E's units */
+	const float hx = fmaf(e.Sxxx, uu, fmaf(e.Sxyy, vv, fmaf(e.Sxzz, qq,
+			fmaf(e.Sxxy, uv, fmaf(e.Sxxz, uq, e.Sxyz * vq)))));
+	const float hy = fmaf(e.Sxxy, uu, fmaf(e.Syyy, vv, fmaf(e.Syzz, qq,
+			fmaf(e.Sxyy, uv, fmaf(e.Sxyz, uq, e.Syyz * vq)))));
+	const float hz = fmaf(e.Sxxz, uu, fmaf(e.Syyz, vv, fmaf(e.Szzz, qq,
+			fmaf(e.Sxyz, uv, fmaf(e.Sxzz, uq, e.Syzz * vq)))));
+	e.ex = fmaf(w.taylor2, hx, fmaf(Jxx, c.du, fmaf(Jxy, c.dv,
+			fmaf(Jxz, c.dq, r0.x))));
+	e.ey = fmaf(w.taylor2, hy, fmaf(Jxy, c.du, fmaf(Jyy, c.dv,
+			fmaf(Jyz, c.dq, r0.y))));
+	e.ez = fmaf(w.taylor2, hz, fmaf(Jxz, c.du, fmaf(Jyz, c.dv,
+			fmaf(Jzz, c.dq, r0.z))));
+	return e;
+}
+
+/* Ewald_Correction (Gravity/periodic.c:35-113) of a point mass m' */
+__device__ __forceinline__ void ewald16(Acc32 &a, const float4 *__restrict__ t,
+		float m, float dx, float dy, float dz, const Walk32Params &w)
+{
+	const EwaldNode c = ewald_node(dx, dy, dz, w);
+	const EwaldLocal e = ewald_local(t, c, w);
+	a.x = fmaf(m * c.sx, e.ex, a.x);
+	a.y = fmaf(m * c.sy, e.ey, a.y);
+	a.z = fmaf(m * c.sz, e.ez, a.z);
+	a.ewald++;
+}
+
+/* The correction of a node of mass m' and traceless second moment T'
+ * {Txx, Tyy, Txy, Txz}, Tyz about its centre of mass at d, to second order:
+ * m' E(d) + Q:S(d) with Q:S = T:S/6 (S = d d E is symmetric in its three
+ * indices and traceless in the last two, so the trace part of the second
+ * moment adds nothing), S taken at the nearest node.  The reflection signs
+ * (S_abc is odd in each axis that appears an odd number of times) are
+ * folded into the coefficients. */
 __device__ __forceinline__ void ewald16q(Acc32 &a, const float4 *__restrict__ t,
 		float m, const float4 qa, float qyz, float dx, float dy, float dz,
 		const Walk32Params &w)
 {
-	const EwaldCell c = ewald_cell(dx, dy, dz, w);
+	const EwaldNode c = ewald_node(dx, dy, dz, w);
+	const EwaldLocal e = ewald_local(t, c, w);
 	/* T' (masses G S^2, lengths S) to table units: / (S^4 Box^4), in two
 	 * steps so no constant is subnormal */
-	const float tx = qa.x * w.inv_s2 * w.inv_s2_box4,
-			ty = qa.y * w.inv_s2 * w.inv_s2_box4;
-	const float tri = m * w.tri_corr;
-	const float cxx = fmaf(-tri, c.fu * (1.f - c.fu), tx);
-	const float cyy = fmaf(-tri, c.fv * (1.f - c.fv), ty);
-	const float czz = fmaf(-tri, c.fq * (1.f - c.fq), -(tx + ty));
+	const float cxx = qa.x * w.inv_s2 * w.inv_s2_box4,
+			cyy = qa.y * w.inv_s2 * w.inv_s2_box4;
+	const float czz = -(cxx + cyy);
 	const float t2 = 2.f * w.inv_s2 * w.inv_s2_box4;
 	const float cxy = qa.z * t2 * (c.sx * c.sy), cxz = qa.w * t2 * (c.sx * c.sz),
 			cyz = qyz * t2 * (c.sy * c.sz);
-	float ax = 0.f, ay = 0.f, az = 0.f;
-#pragma unroll 2
-	for (int n = 0; n < 8; n++) {
-		const int di = n >> 2, dj = (n >> 1) & 1, dk = n & 1;
-		const float wt = (di ? c.fu : 1.f - c.fu) * (dj ? c.fv : 1.f - c.fv)
-				* (dk ? c.fq : 1.f - c.fq);
-		const float4 *r = t + 2 * (c.b + (di * TDV_NE1 + dj) * TDV_NE1 + dk);
-		const float4 r0 = r[0], r1 = r[1];
-		const float Sxxx = h2lo(r0.w), Syyy = h2hi(r0.w), Szzz = h2lo(r1.x),
-				Sxxy = h2hi(r1.x), Sxxz = h2lo(r1.y), Sxyy = h2hi(r1.y),
-				Syyz = h2lo(r1.z), Sxzz = h2hi(r1.z), Syzz = h2lo(r1.w),
-				Sxyz = h2hi(r1.w);
-		float vx = m * r0.x, vy = m * r0.y, vz = m * r0.z;
-		vx = fmaf(cxx, Sxxx, fmaf(cyy, Sxyy, fmaf(czz, Sxzz, vx)));
-		vx = fmaf(cxy, Sxxy, fmaf(cxz, Sxxz, fmaf(cyz, Sxyz, vx)));
-		vy = fmaf(cxx, Sxxy, fmaf(cyy, Syyy, fmaf(czz, Syzz, vy)));
-		vy = fmaf(cxy, Sxyy, fmaf(cxz, Sxyz, fmaf(cyz, Syyz, vy)));
-		vz = fmaf(cxx, Sxxz, fmaf(cyy, Syyz, fmaf(czz, Szzz, vz)));
-		vz = fmaf(cxy, Sxyz, fmaf(cxz, Sxzz, fmaf(cyz, Syzz, vz)));
-		ax = fmaf(wt, vx, ax);
-		ay = fmaf(wt, vy, ay);
-		az = fmaf(wt, vz, az);
-	}
-	a.x = fmaf(c.sx, ax, a.x);
-	a.y = fmaf(c.sy, ay, a.y);
-	a.z = fmaf(c.sz, az, a.z);
+	float vx = m * e.ex, vy = m * e.ey, vz = m * e.ez;
+	vx = fmaf(cxx, e.Sxxx, fmaf(cyy, e.Sxyy, fmaf(czz, e.Sxzz, vx)));
+	vx = fmaf(cxy, e.Sxxy, fmaf(cxz, e.Sxxz, fmaf(cyz, e.Sxyz, vx)));
+	vy = fmaf(cxx, e.Sxxy, fmaf(cyy, e.Syyy, fmaf(czz, e.Syzz, vy)));
+	vy = fmaf(cxy, e.Sxyy, fmaf(cxz, e.Sxyz, fmaf(cyz, e.Syyz, vy)));
+	vz = fmaf(cxx, e.Sxxz, fmaf(cyy, e.Syyz, fmaf(czz, e.Szzz, vz)));
+	vz = fmaf(cxy, e.Sxyz, fmaf(cxz, e.Sxzz, fmaf(cyz, e.Syzz, vz)));
+	a.x = fmaf(c.sx, vx, a.x);
+	a.y = fmaf(c.sy, vy, a.y);
+	a.z = fmaf(c.sz, vz, a.z);
 	a.ewald++;
 }
 
@@ -1129,18 +1153,17 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	if (c->par.periodic && !c->ewald16_ok) {
 		if (!c->have_ewald)
 			return tdv_fail(c, TDV_ERR_ARG, "FP32 walk without an Ewald table");
-		if (!c->ewald_s) {
-			TDV_HIP(c, hipMalloc((void **)&c->ewald_s,
-					10 * (size_t)TDV_NE3 * sizeof(double)));
-			TDV_TRY(tdv_ewald_derivs_device(c, c->ewald_s));
-		}
+		TDV_TRY(tdv_ewald_derivs_ready(c));
 		if (!c->ewald16)
 			TDV_HIP(c, hipMalloc((void **)&c->ewald16,
-					2 * (size_t)TDV_NE3 * sizeof(float4)));
-		/* E: c->ewald is F / Box^2; m' = G m S^2, so E / S^2 */
-		hipLaunchKernelGGL(k_prep32_ewald16, dim3(tdv_blocks(TDV_NE3, 256)),
-				dim3(256), 0, c->stream, c->ewald, c->ewald_s, 1.0 / (S * S),
-				c->ewald16);
+					4 * (size_t)TDV_NE3 * sizeof(float4)));
+		/* E: c->ewald is F / Box^2; m' = G m S^2, so E / S^2.  J: per unit
+		 * box -> per grid unit (1 / (2 N_EWALD)), / Box^2 / S^2 */
+		const double box = c->par.boxsize;
+		hipLaunchKernelGGL(k_prep32_ewald_taylor,
+				dim3(tdv_blocks(TDV_NE3, 256)), dim3(256), 0, c->stream,
+				c->ewald, c->ewald_s, c->ewald_j, 1.0 / (S * S),
+				1.0 / (2.0 * TDV_NE * box * box * S * S), c->ewald16);
 		TDV_HIP(c, hipGetLastError());
 		c->ewald16_ok = 1;
 	}
@@ -1170,11 +1193,11 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	w.inv_box_f = 1.f / w.box_f;
 	w.inv_s2 = (float)(1.0 / (S * S));
 	{
-		/* u(1-u) h^2 (d_k^2 E) / 2 in E's table units (E / (Box^2 S^2)):
-		 * 3 h_u^2 S_half / (Box^2 S^2) with h_u = 1/128, S_half = S / 6 */
+		/* (1/2) S per grid unit = 3 (S/6) h^2 with h = 1/128 box, in E's
+		 * units (E / (Box^2 S^2)) */
 		const double box = c->par.periodic ? c->par.boxsize : 1.0;
 		const double hu = 1.0 / (2.0 * TDV_NE);
-		w.tri_corr = (float)(3.0 * hu * hu / (box * box * S * S));
+		w.taylor2 = (float)(3.0 * hu * hu / (box * box * S * S));
 		w.inv_s2_box4 = (float)(1.0 / (S * S * box * box * box * box));
 	}
 	w.carry_every = c->par.fp32_carry > 0 ? c->par.fp32_carry : 16;

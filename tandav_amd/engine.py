@@ -106,6 +106,7 @@ def load_library(path=LIB_PATH):
     L.tdv_set_active.argtypes = [_ctx, ctypes.c_int, _vp]
     L.tdv_set_tree_moments.argtypes = [_ctx, _vp, _sz, _vp, ctypes.c_int]
     L.tdv_get_ewald_derivs.argtypes = [_ctx, _dp]
+    L.tdv_get_ewald_jacobian.argtypes = [_ctx, _dp]
     L.tdv_get_walk32_quadrupoles.argtypes = [_ctx, _vp, _vp, _vp]
     L.tdv_kick_drift.argtypes = [_ctx, ctypes.c_double, ctypes.c_double]
     L.tdv_set_fp32_options.argtypes = [_ctx, ctypes.c_int, ctypes.c_int,
@@ -414,6 +415,14 @@ class Engine:
         self._ok(self.lib.tdv_get_ewald_derivs(self._c, _d(s)),
                  "tdv_get_ewald_derivs")
         return s
+
+    def ewald_jacobian(self):
+        """First derivatives d_b E_a of the Ewald correction at the table
+        nodes, (6, 65, 65, 65), per unit box, components xx yy zz xy xz yz."""
+        j = np.zeros((6, 65, 65, 65))
+        self._ok(self.lib.tdv_get_ewald_jacobian(self._c, _d(j)),
+                 "tdv_get_ewald_jacobian")
+        return j
 
     def set_ewald_table(self, t):
         t = np.ascontiguousarray(t, dtype=np.float64)

@@ -138,6 +138,31 @@ def test_ewald_second_derivatives(table):
     assert np.abs(S[:, 0, 0, 0]).max() < 1e-6 * scale
 
 
+def test_ewald_first_derivatives(table):
+    """The first derivatives J_ab = d_b E_a the FP32 walk expands the Ewald
+    correction with: against central differences of the oracle's FP64
+    table, symmetric by construction, and with the divergence the closed
+    form fixes: away from the particle the periodic force of a unit mass in
+    a unit box has div = 4 pi (the uniform background), the nearest-image
+    Newton force none, so div E = -4 pi for E = -(that difference) at
+    every node, origin and faces included (to the truncation of the
+    reference's sums, alpha 2 with |n|, |h| <= 4: ~7e-5 of 4 pi)."""
+    e = Engine(periodic=True, softening=1e-3)
+    J = e.ewald_jacobian()
+    E = -table[:3]
+    h = 0.5 / 64
+    sl = (slice(2, 62),) * 3
+    comp = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]   # (E comp, axis)
+    scale = np.abs(J[:, 2:62, 2:62, 2:62]).max()
+    for q, (c, ax) in enumerate(comp):
+        fd = (np.roll(E[c], -1, ax) - np.roll(E[c], 1, ax)) / (2 * h)
+        assert np.abs(fd[sl] - J[q][sl]).max() < 2e-3 * scale, q
+    tr = J[0] + J[1] + J[2]
+    np.testing.assert_allclose(tr, -4 * np.pi, rtol=0, atol=2e-4 * 4 * np.pi)
+    # E is odd in x: J_xy vanishes on the x = 0 face, J_xx does not
+    assert np.abs(J[3][0]).max() < 1e-9 * scale
+
+
 def test_forcetest_exact_on_table_nodes(table):
     """The exact-Ewald direct sum against the oracle's table: separations
     placed on table nodes, where trilinear interpolation returns the node
