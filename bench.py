@@ -546,7 +546,11 @@ def main():
 
     head = res["fp32"] if "fp32" in res else res["fp64"]
     if "fp32" in res:
-        kern = "k_walk32"
+        # the kernel the timed walk ran (the autotuner's pick, full set)
+        pe = head["e"].params
+        kern = ("k_walk32_queue" if pe.fp32_queue else
+                {0: "k_walk32", 6: "k_walk32_w6",
+                 8: "k_walk32_w8"}[pe.fp32_waves])
         t_k = head["stage"]["walk"]
         flops = (head["newton"] * FLOPS_PER_INTERACTION
                  + head["ewald"] * FLOPS_PER_EWALD)

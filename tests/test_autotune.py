@@ -63,3 +63,11 @@ def test_group_sample_takes_whole_groups():
     assert np.all(np.diff(idx) > 0)
     np.testing.assert_array_equal(at._group_sample(300, 7, seed=1),
                                   np.arange(300))
+
+
+def test_keeps_default_within_margin():
+    def cost(w, q, s):             # 1% faster than the default: noise
+        return 10.0 - (0.1 if (w, q) == (8, 1) else 0.0)
+    e = FakeEngine(500, cost)
+    r = at.autotune(e, repeats=1)
+    assert (r["waves"], r["queue"], r["split"]) == (0, 0, 0)
