@@ -274,6 +274,15 @@ int tdv_set_active(tdv_context *ctx, int nactive, const int *list);
 int tdv_set_tree_moments(tdv_context *ctx, const void *tree, size_t nnodes,
 		const void *top, int ntop);
 
+/* Make_Active_Particle_List (timestep.c:280-299) on the device: the
+ * particles with Time_Bin <= max_active_bin, in particle order, become the
+ * active list of the next walk (and the device count captured walks read).
+ * time_bin, if not NULL, first replaces the context's P.Time_Bin column (n
+ * ints, current particle order).  No active particle is an error, as the
+ * reference's Assert. */
+int tdv_make_active_list(tdv_context *ctx, int max_active_bin,
+		const int *time_bin);
+
 /* The FP32 walk's launch options of an existing context (the tdv_params
  * fields fp32_waves, fp32_queue, fp32_split, same ranges): they change the
  * schedule, not the interactions, so an autotuner can time them on one tree.

@@ -1212,6 +1212,52 @@ int tdv_interaction_counts(tdv_context *c, uint64_t counts[2])
 	return 0;
 }
 
+__global__ void k_active_flags(size_t n, const int *__restrict__ bin,
+		int max_bin, uint32_t *__restrict__ flag)
+{
+	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		flag[i] = bin[i] <= max_bin ? 1u : 0u;
+}
+
+__global__ void k_active_scatter(size_t n, const uint32_t *__restrict__ flag,
+		const uint32_t *__restrict__ off, int *__restrict__ active,
+		int *__restrict__ nact)
+{
+	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n && flag[i])
+		active[off[i]] = (int)i;
+	if (i == 0)
+		*nact = (int)off[n];
+}
+
+int tdv_make_active_list(tdv_context *c, int max_active_bin,
+		const int *time_bin)
+{
+	if (!c || !c->n)
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_make_active_list: no particles");
+	const size_t n = c->n;
+	if (time_bin)
+		TDV_HIP(c, hipMemcpyAsync(c->time_bin, time_bin, n * sizeof(int),
+				hipMemcpyHostToDevice, c->stream));
+	uint32_t *flag = reinterpret_cast<uint32_t *>(c->cnt);
+	hipLaunchKernelGGL(k_active_flags, dim3(tdv_blocks(n, 256)), dim3(256), 0,
+			c->stream, n, c->time_bin, max_active_bin, flag);
+	TDV_TRY(tdv_scan_u32(c, flag, c->off, n));
+	hipLaunchKernelGGL(k_active_scatter, dim3(tdv_blocks(n, 256)), dim3(256),
+			0, c->stream, n, flag, c->off, c->active, c->d_nactive);
+	TDV_HIP(c, hipGetLastError());
+	int nact = 0;
+	TDV_HIP(c, hipMemcpyAsync(&nact, c->d_nactive, sizeof(int),
+			hipMemcpyDeviceToHost, c->stream));
+	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	c->nactive = nact;
+	if (nact <= 0)                   /* timestep.c:296-297 */
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_make_active_list: no active "
+				"particles, bin max " + std::to_string(max_active_bin));
+	return 0;
+}
+
 int tdv_set_fp32_options(tdv_context *c, int waves, int queue, int split)
 {
 	if (!c || (waves != 0 && waves != 6 && waves != 8)

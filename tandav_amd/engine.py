@@ -109,6 +109,7 @@ def load_library(path=LIB_PATH):
     L.tdv_get_ewald_jacobian.argtypes = [_ctx, _dp]
     L.tdv_get_walk32_quadrupoles.argtypes = [_ctx, _vp, _vp, _vp]
     L.tdv_kick_drift.argtypes = [_ctx, ctypes.c_double, ctypes.c_double]
+    L.tdv_make_active_list.argtypes = [_ctx, ctypes.c_int, _vp]
     L.tdv_set_fp32_options.argtypes = [_ctx, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int]
     L.tdv_capture_walk.argtypes = [_ctx]
@@ -458,6 +459,16 @@ class Engine:
                                        n.value, ctypes.byref(n)),
                  "tdv_get_tree")
         return buf
+
+    def make_active_list(self, max_active_bin, time_bin=None):
+        """Make_Active_Particle_List on the device: Time_Bin <= max bin
+        (after replacing the Time_Bin column with `time_bin`, if given)."""
+        tb = (None if time_bin is None
+              else np.ascontiguousarray(time_bin, dtype=np.int32))
+        self._ok(self.lib.tdv_make_active_list(
+            self._c, int(max_active_bin),
+            tb.ctypes.data_as(_vp) if tb is not None else None),
+            "tdv_make_active_list")
 
     def set_fp32_options(self, waves=None, queue=None, split=None):
         """Change the FP32 walk's launch options (register budget, work-queue

@@ -103,3 +103,32 @@ def test_split_walk_matches_whole_walk(case):
         assert np.median(err) < 1e-5, (case, split, len(sub), np.median(err))
         assert err.max() < 1e-4, (case, split, len(sub), err.max())
         parts.close()
+
+
+def test_device_active_list_matches_host_list():
+    """Make_Active_Particle_List on the device (Time_Bin <= max bin, in
+    particle order) drives the eager walk and a captured graph exactly as
+    the host-made list does; an empty list is an error, as in Tandav."""
+    from tandav_amd.engine import TDVError
+    pos, _, m = ics.zeldovich_box(32, seed=7)
+    n = len(m)
+    a = _engine(pos, m, 1)
+    b = _engine(pos, m, 1)
+    st = a.get("pos", "acc", "last_acc_mag")
+    bins = np.random.Generator(np.random.Philox(9)).integers(
+        1, 6, n).astype(np.int32)
+    b.capture_walk()
+    for mb in (1, 3, 5):
+        act = np.flatnonzero(bins <= mb).astype(np.int32)
+        a.set_active(act)
+        for e in (a, b):
+            e.update_particles(st["pos"], acc=st["acc"],
+                               last_acc_mag=st["last_acc_mag"])
+        b.make_active_list(mb, time_bin=bins)
+        a.compute_acceleration(False)
+        b.launch_walk()
+        ga, gb = a.get("acc", "cost"), b.get("acc", "cost")
+        np.testing.assert_array_equal(ga["acc"], gb["acc"])
+        np.testing.assert_array_equal(ga["cost"], gb["cost"])
+    with pytest.raises(TDVError):
+        b.make_active_list(0)
