@@ -14,6 +14,7 @@
  */
 #include "tdv_internal.h"
 #include "tdv_device.h"
+#include <climits>
 
 struct WalkParams {
 	int ntop, vs, compat, periodic, use_bh;
@@ -66,11 +67,17 @@ __device__ __forceinline__ void interact(Acc &a, double m, double dx,
 	a.cost += 1;
 }
 
-__device__ __forceinline__ void pp_range(Acc &a, const double4 *pm, int first,
-		int last, double px, double py, double pz, const WalkParams &w)
+/* P-P with the particles [first, last) for the lanes with `on`, in the
+ * reference's order; the range is the wavefront's, so the loads are
+ * uniform */
+__device__ __forceinline__ void pp_range_on(Acc &a, const double4 *pm,
+		int first, int last, double px, double py, double pz, bool on,
+		const WalkParams &w)
 {
 	for (int j = first; j < last; j++) {
 		const double4 q = pm[j];
+		if (!on)
+			continue;
 		const double dx = wrap(q.x - px, w), dy = wrap(q.y - py, w),
 				dz = wrap(q.z - pz, w);
 		const double r2 = dx * dx + dy * dy + dz * dz;
@@ -79,6 +86,25 @@ __device__ __forceinline__ void pp_range(Acc &a, const double4 *pm, int first,
 	}
 }
 
+/* the smallest value over the wavefront, as a wave-uniform value */
+__device__ __forceinline__ int wave_min(int v)
+{
+	for (int d = 32; d >= 1; d >>= 1) {
+		const int o = __shfl_xor(v, d, 64);
+		v = o < v ? o : v;
+	}
+	return __builtin_amdgcn_readfirstlane(v);
+}
+
+/* One lane per active particle, every lane taking exactly the reference's
+ * walk: its own decisions, its own interactions in its own order.  The
+ * wavefront walks in lockstep: each lane keeps the next node of its walk,
+ * the wavefront takes the smallest of them (walks only move forward in
+ * the depth-first array), loads that node once through the scalar cache,
+ * and the lanes whose next node it is act on it; the others wait.  PH
+ * neighbours' walks share most nodes, so the wavefront visits little more
+ * than one walk's nodes, without the per-lane gathers and divergent
+ * branches of independent loops. */
 __global__ void __launch_bounds__(256) k_walk64(int nact,
 		const int *__restrict__ active, const double4 *__restrict__ pm,
 		const double *__restrict__ last_acc, TopNodes T,
@@ -96,15 +122,17 @@ __global__ void __launch_bounds__(256) k_walk64(int nact,
 	const double fac = last_acc[ip] / w.G * w.alpha_rel;
 	Acc a = { 0, 0, 0, 0 };
 
-	for (int j = 0; live && j < w.ntop; j++) {
+	for (int j = 0; j < w.ntop; j++) {
 		/* interact_with_topnode, Gravity/tree_accel.c:137-176 */
 		const double nSize = w.dom_size / ((double)(1ull << T.level[j]));
-		{
+		bool walk = false;
+		if (live) {
 			const double dx = (double)T.pos[3 * j] - p.x;
 			const double dy = (double)T.pos[3 * j + 1] - p.y;
 			const double dz = (double)T.pos[3 * j + 2] - p.z;
 			const bool inside = fabs(dx) < 0.6 * nSize && fabs(dy) < 0.6 * nSize
 					&& fabs(dz) < 0.6 * nSize;
+			walk = inside;
 			if (!inside) {
 				const double rx = wrap((double)T.com[3 * j] - p.x, w);
 				const double ry = wrap((double)T.com[3 * j + 1] - p.y, w);
@@ -113,35 +141,44 @@ __global__ void __launch_bounds__(256) k_walk64(int nact,
 				const double m = (double)T.mass[j];
 				const bool open = w.use_bh ? (nSize * nSize > r2 * w.theta_bh)
 						: (m * nSize * nSize > r2 * r2 * fac);
-				if (!open) {
+				if (open)
+					walk = true;
+				else
 					interact(a, m, rx, ry, rz, r2, w);
-					continue;
-				}
 			}
 		}
+		if (!__any(walk))
+			continue;
 		if (T.npart[j] <= w.vs) {
 			/* interact_with_topnode_particles, tree_accel.c:183-203 */
 			const int f = w.compat ? T.target[j] : T.first[j];
 			if (f < 0 || f + T.npart[j] > npart_total) {
-				atomicOr(err, TDV_EBIT_RANGE);
+				if (walk)
+					atomicOr(err, TDV_EBIT_RANGE);
 				continue;
 			}
-			pp_range(a, pm, f, f + T.npart[j], p.x, p.y, p.z, w);
+			pp_range_on(a, pm, f, f + T.npart[j], p.x, p.y, p.z, walk, w);
 			continue;
 		}
 		/* gravity_tree_walk(_BH), Gravity/tree_accel.c:211-353 */
 		const int start = T.target[j], term = T.term[j];
-		int node = start;
+		int next = walk ? start : INT_MAX;
 		for (;;) {
-			if (node < start || node > term) {   /* never for a valid tree */
+			const int node = wave_min(next);
+			if (node == INT_MAX)
+				break;
+			if (node < start || node > term) {  /* never for a valid tree */
 				atomicOr(err, TDV_EBIT_RANGE);
 				break;
 			}
+			const bool mine = next == node;
 			const int4 c = nc[node];
-			if (c.x == 0) {                      /* the terminator */
+			if (c.x == 0) {                     /* the terminator */
 				if (node == start)
 					atomicOr(err, TDV_EBIT_RANGE);
-				break;
+				if (mine)
+					next = INT_MAX;
+				continue;
 			}
 			if (c.x < 0) {
 				const int f = -c.x - 1;
@@ -149,37 +186,38 @@ __global__ void __launch_bounds__(256) k_walk64(int nact,
 					atomicOr(err, TDV_EBIT_RANGE);
 					break;
 				}
-				pp_range(a, pm, f, f + c.y, p.x, p.y, p.z, w);
-				node++;
+				pp_range_on(a, pm, f, f + c.y, p.x, p.y, p.z, mine, w);
+				if (mine)
+					next = node + 1;
 				continue;
 			}
 			const double4 A = na[node];
+			const double4 B = nb[node];
+			if (!mine)
+				continue;
 			const double rx = wrap(A.x - p.x, w), ry = wrap(A.y - p.y, w),
 					rz = wrap(A.z - p.z, w);
 			const double r2 = rx * rx + ry * ry + rz * rz;
 			const double m = A.w;
-			const double4 B = nb[node];
 			const double s = B.w;
+			bool open;
 			if (w.use_bh) {
-				if (s * s > r2 * w.theta_bh) {
-					node++;
-					continue;
-				}
+				open = s * s > r2 * w.theta_bh;
 			} else {
-				if (m * s * s > r2 * r2 * fac) {
-					node++;
-					continue;
-				}
-				const double sx = wrap(B.x - p.x, w), sy = wrap(B.y - p.y, w),
-						sz = wrap(B.z - p.z, w);
-				if (fabs(sx) < 0.6 * s && fabs(sy) < 0.6 * s
-						&& fabs(sz) < 0.6 * s) {
-					node++;
-					continue;
+				open = m * s * s > r2 * r2 * fac;
+				if (!open) {
+					const double sx = wrap(B.x - p.x, w),
+							sy = wrap(B.y - p.y, w), sz = wrap(B.z - p.z, w);
+					open = fabs(sx) < 0.6 * s && fabs(sy) < 0.6 * s
+							&& fabs(sz) < 0.6 * s;
 				}
 			}
-			interact(a, m, rx, ry, rz, r2, w);
-			node += c.x;
+			if (open) {
+				next = node + 1;
+			} else {
+				interact(a, m, rx, ry, rz, r2, w);
+				next = node + c.x;
+			}
 		}
 	}
 	/* add_recv_to, Gravity/tree_accel.c:110-129 */
@@ -249,12 +287,14 @@ __device__ __forceinline__ void ewald_cube(Acc &a, const double *tab,
 	a.cost += 1;
 }
 
-__device__ __forceinline__ void ewald_pp_range(Acc &a, const double *tab,
+__device__ __forceinline__ void ewald_pp_range_on(Acc &a, const double *tab,
 		const double4 *pm, int first, int last, double px, double py,
-		double pz, const WalkParams &w)
+		double pz, bool on, const WalkParams &w)
 {
 	for (int j = first; j < last; j++) {
 		const double4 q = pm[j];
+		if (!on)
+			continue;
 		const double dx = wrap(q.x - px, w), dy = wrap(q.y - py, w),
 				dz = wrap(q.z - pz, w);
 		if (dx != 0 && dy != 0 && dz != 0)
@@ -262,6 +302,7 @@ __device__ __forceinline__ void ewald_pp_range(Acc &a, const double *tab,
 	}
 }
 
+/* Gravity_Tree_Periodic per lane, in lockstep as k_walk64 */
 __global__ void __launch_bounds__(256) k_walk64_ewald(int nact,
 		const int *__restrict__ active, const double4 *__restrict__ pm,
 		const double *__restrict__ last_acc, TopNodes T,
@@ -280,61 +321,70 @@ __global__ void __launch_bounds__(256) k_walk64_ewald(int nact,
 	const double fac = last_acc[ip] / w.G * w.alpha_rel;
 	Acc a = { 0, 0, 0, 0 };
 
-	for (int j = 0; live && j < w.ntop; j++) {
+	for (int j = 0; j < w.ntop; j++) {
 		/* interact_with_topnode, Gravity/tree_periodic.c:112-177 */
 		const double s = w.dom_size / (double)(1ull << T.level[j]);
-		bool want_open = false;
-		const double rx = wrap((double)T.com[3 * j] - p.x, w);
-		const double ry = wrap((double)T.com[3 * j + 1] - p.y, w);
-		const double rz = wrap((double)T.com[3 * j + 2] - p.z, w);
-		const double r2 = rx * rx + ry * ry + rz * rz;
-		if (w.use_bh) {
-			if (s * s > r2 * w.theta_bh)
-				want_open = true;
-		} else {
-			const double m = (double)T.mass[j];
-			if (m * s * s > r2 * r2 * fac)
-				want_open = true;
-		}
-		if (fabs(p.x - (double)T.pos[3 * j]) < 0.6 * s
-				&& fabs(p.y - (double)T.pos[3 * j + 1]) < 0.6 * s
-				&& fabs(p.z - (double)T.pos[3 * j + 2]) < 0.6 * s)
-			want_open = true;
-		/* a node that wants opening is walked only when it straddles the
-		 * half box or is larger than 0.2 box; otherwise the correction of
-		 * its centre of mass is used */
 		bool walk = false;
-		if (want_open) {
-			const double lim = 0.5 * (w.box - s);
-			walk = fabs(rx) > lim || fabs(ry) > lim || fabs(rz) > lim
-					|| s > 0.2 * w.box;
+		if (live) {
+			bool want_open = false;
+			const double rx = wrap((double)T.com[3 * j] - p.x, w);
+			const double ry = wrap((double)T.com[3 * j + 1] - p.y, w);
+			const double rz = wrap((double)T.com[3 * j + 2] - p.z, w);
+			const double r2 = rx * rx + ry * ry + rz * rz;
+			if (w.use_bh) {
+				if (s * s > r2 * w.theta_bh)
+					want_open = true;
+			} else {
+				const double m = (double)T.mass[j];
+				if (m * s * s > r2 * r2 * fac)
+					want_open = true;
+			}
+			if (fabs(p.x - (double)T.pos[3 * j]) < 0.6 * s
+					&& fabs(p.y - (double)T.pos[3 * j + 1]) < 0.6 * s
+					&& fabs(p.z - (double)T.pos[3 * j + 2]) < 0.6 * s)
+				want_open = true;
+			/* a node that wants opening is walked only when it straddles
+			 * the half box or is larger than 0.2 box; otherwise the
+			 * correction of its centre of mass is used */
+			if (want_open) {
+				const double lim = 0.5 * (w.box - s);
+				walk = fabs(rx) > lim || fabs(ry) > lim || fabs(rz) > lim
+						|| s > 0.2 * w.box;
+			}
+			if (!walk)
+				ewald_cube(a, tab, rx, ry, rz, (double)T.mass[j], w);
 		}
-		if (!walk) {
-			ewald_cube(a, tab, rx, ry, rz, (double)T.mass[j], w);
+		if (!__any(walk))
 			continue;
-		}
 		if (T.npart[j] <= w.vs) {
 			const int f = w.compat ? T.target[j] : T.first[j];
 			if (f < 0 || f + T.npart[j] > npart_total) {
-				atomicOr(err, TDV_EBIT_RANGE);
+				if (walk)
+					atomicOr(err, TDV_EBIT_RANGE);
 				continue;
 			}
-			ewald_pp_range(a, tab, pm, f, f + T.npart[j], p.x, p.y, p.z, w);
+			ewald_pp_range_on(a, tab, pm, f, f + T.npart[j], p.x, p.y, p.z,
+					walk, w);
 			continue;
 		}
 		/* gravity_tree_walk_ewald(_BH), Gravity/tree_periodic.c:216-422 */
 		const int start = T.target[j];
 		const int end = start + nc[start].x;
 		if (end != T.term[j]) {
-			atomicOr(err, TDV_EBIT_RANGE);
+			if (walk)
+				atomicOr(err, TDV_EBIT_RANGE);
 			continue;
 		}
-		int node = start;
-		while (node != end) {
-			if (node < start || node > end) {    /* never for a valid tree */
+		int next = walk ? start : INT_MAX;
+		for (;;) {
+			const int node = wave_min(next);
+			if (node == INT_MAX)
+				break;
+			if (node < start || node >= end) {   /* never for a valid tree */
 				atomicOr(err, TDV_EBIT_RANGE);
 				break;
 			}
+			const bool mine = next == node;
 			const int4 c = nc[node];
 			if (c.x < 0) {
 				const int f = -c.x - 1;
@@ -342,16 +392,20 @@ __global__ void __launch_bounds__(256) k_walk64_ewald(int nact,
 					atomicOr(err, TDV_EBIT_RANGE);
 					break;
 				}
-				ewald_pp_range(a, tab, pm, f, f + c.y, p.x, p.y, p.z, w);
-				node++;
+				ewald_pp_range_on(a, tab, pm, f, f + c.y, p.x, p.y, p.z, mine,
+						w);
+				if (mine)
+					next = node + 1 == end ? INT_MAX : node + 1;
 				continue;
 			}
 			const double4 A = na[node];
+			const double4 B = nb[node];
+			if (!mine)
+				continue;
 			const double dx = wrap(A.x - p.x, w), dy = wrap(A.y - p.y, w),
 					dz = wrap(A.z - p.z, w);
 			const double d2 = dx * dx + dy * dy + dz * dz;
 			const double m = A.w;
-			const double4 B = nb[node];
 			const double ns = B.w;
 			const double sx = B.x - p.x, sy = B.y - p.y, sz = B.z - p.z;
 			bool open = w.use_bh ? (ns * ns > d2 * w.theta_bh2)
@@ -359,16 +413,22 @@ __global__ void __launch_bounds__(256) k_walk64_ewald(int nact,
 			if (!open && fabs(sx) < 0.6 * ns && fabs(sy) < 0.6 * ns
 					&& fabs(sz) < 0.6 * ns)
 				open = true;
+			int to;
 			if (open) {
 				const double lim = 0.5 * (w.box - ns);
 				if (fabs(wrap(sx, w)) > lim || fabs(wrap(sy, w)) > lim
-						|| fabs(wrap(sz, w)) > lim || ns > 0.2 * w.box) {
-					node++;
-					continue;
-				}
+						|| fabs(wrap(sz, w)) > lim || ns > 0.2 * w.box)
+					to = node + 1;
+				else
+					to = -1;
+			} else {
+				to = -1;
 			}
-			ewald_cube(a, tab, dx, dy, dz, m, w);
-			node += c.x;
+			if (to < 0) {
+				ewald_cube(a, tab, dx, dy, dz, m, w);
+				to = node + c.x;
+			}
+			next = to == end ? INT_MAX : to;
 		}
 	}
 	/* add_recv_to, Gravity/tree_periodic.c:91-110 */
