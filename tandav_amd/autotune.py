@@ -11,8 +11,8 @@ pick per workload is the schedule:
 * ``split``: for substeps with few active groups, how many parts each
   group's subtree walks may be cut into (1 = never; n >= 2 = up to n).
 
-The default schedule (compiler's budget, one launch per group) is kept
-unless another beats it by more than MARGIN.
+Of the candidates within MARGIN of the fastest, the first in the order
+above is kept (the default schedule first), so noise does not flip it.
 
 None of them changes the interactions a particle takes (waves and queue not
 even the rounding), so they are timed on one tree: the walk of the full
@@ -28,7 +28,7 @@ WAVES = (0, 6, 8)
 QUEUE = (0, 1)
 SPLITS = (1, 8, 16, 32, 64, 128, 256)
 SAMPLE_GROUPS = 65536
-MARGIN = 0.03          # another schedule must beat the default by this much
+MARGIN = 0.03          # candidates this close to the fastest count as equal
 
 
 def _trial(e, state, active, repeats):
@@ -69,11 +69,11 @@ def autotune(e, active_sets=(), repeats=2, sample_groups=SAMPLE_GROUPS,
             trials["full"].append({"waves": w, "queue": q,
                                    "walk_ms": _trial(e, state, full,
                                                      repeats)})
-    best = min(trials["full"], key=lambda r: r["walk_ms"])
-    default = next(r for r in trials["full"]
-                   if r["waves"] == 0 and r["queue"] == 0)
-    if best["walk_ms"] > (1.0 - MARGIN) * default["walk_ms"]:
-        best = default                 # within the timing noise: keep it
+    # the first candidate (the default first) within MARGIN of the fastest:
+    # choices inside the timing noise do not flip from run to run
+    fastest = min(r["walk_ms"] for r in trials["full"])
+    best = next(r for r in trials["full"]
+                if r["walk_ms"] <= (1.0 + MARGIN) * fastest)
     split = 0
     sets = [np.asarray(a, dtype=np.int32) for a in active_sets if len(a)]
     if sets:

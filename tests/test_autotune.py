@@ -71,3 +71,8 @@ def test_keeps_default_within_margin():
     e = FakeEngine(500, cost)
     r = at.autotune(e, repeats=1)
     assert (r["waves"], r["queue"], r["split"]) == (0, 0, 0)
+    # the first of the candidates within the margin, not the fastest
+    def cost2(w, q, s):
+        return {(0, 0): 10.0, (0, 1): 10.5, (6, 0): 9.0}.get((w, q), 8.9)
+    r = at.autotune(FakeEngine(500, cost2), repeats=1)
+    assert (r["waves"], r["queue"]) == (6, 0)
