@@ -9,8 +9,9 @@
  * Opening decisions use the same FP64 expressions in the same order as the
  * reference, so the interaction set and P.Cost are identical; the build is
  * compiled without FP contraction so every product and sum rounds once.
- * Lanes of a wavefront are PH neighbours, so their walks visit nearly the
- * same nodes and the node loads coalesce to the same cache lines.
+ * Lanes of a wavefront are PH neighbours whose walks visit nearly the same
+ * nodes, so the wavefront walks in lockstep (see k_walk64): node and
+ * particle records are loaded once per wavefront through the scalar cache.
  */
 #include "tdv_internal.h"
 #include "tdv_device.h"
