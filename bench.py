@@ -101,6 +101,34 @@ def run_gpu(pos, m, soft, w, steps, warmup, precision, f32, tune=False):
     return e, (t1 - t0) / steps, {k: v / steps for k, v in stage.items()}
 
 
+# Algorithmic HBM bytes of the memory-bound stages, per particle (and per
+# tree node for the build): keys read 3 positions, write a 16-byte key; the
+# sort is 8 LSD passes over the keys' high 64-bit words with a 4-byte
+# payload (per pass a histogram read of the key, then key + payload read
+# and written) and the permutation's copy; the reorder reads and writes 148
+# payload bytes (key 16, fifteen FP64 columns, three 32-bit columns); the
+# build reads key, position and mass and writes the tree-parent index, and
+# per node its CoM/mass, centre/size, links and DUp.
+STAGE_BYTES = {"keys": 24 + 16, "sort": 8 * (8 + 2 * (8 + 4)) + 2 * 4,
+               "reorder": 2 * (16 + 15 * 8 + 3 * 4),
+               "build": 16 + 24 + 8 + 4}
+BUILD_NODE_BYTES = 32 + 32 + 16 + 4
+
+
+def stage_roofline(stages_ms, n, nnodes):
+    """HBM rate of the keys, sort, reorder and build stages against 8 TB/s
+    from their algorithmic bytes (STAGE_BYTES) and HIP-event times."""
+    out = {}
+    for k, per in STAGE_BYTES.items():
+        ms = stages_ms.get(k, 0.0)
+        b = per * n + (BUILD_NODE_BYTES * nnodes if k == "build" else 0)
+        if ms > 0:
+            out[k] = {"bytes": b, "ms": round(ms, 4),
+                      "GB_s": round(b / (ms * 1e-3) / 1e9, 1),
+                      "frac_of_8TB_s": round(b / (ms * 1e-3) / 8e12, 4)}
+    return out
+
+
 def tune_walk(e, active_sets=()):
     """The autotuner's pick of the FP32 walk's launch options on this tree
     (untimed; tandav_amd/autotune.py)."""
@@ -291,6 +319,8 @@ def bench_c4(args, f32):
                    "error_sample_exact": len(xidx),
                    "parallelism": "single GPU"},
         "full_step": {"stages_ms": {k: round(v, 4) for k, v in stage.items()},
+                      "stage_roofline": stage_roofline(stage, n,
+                                                       e.num_nodes()),
                       "interactions_per_particle": (newton + ewald) / n,
                       "err_vs_exact_ewald": err},
         "substeps": subs,
@@ -403,6 +433,7 @@ def bench_c5(args, f32):
                    "fp32": f32, "kick": kick, "drift": drift,
                    "parallelism": "single GPU"},
         "stages_ms": {k: round(v, 3) for k, v in stage.items()},
+        "stage_roofline": stage_roofline(stage, n, e.num_nodes()),
         "walk_force_evals_per_s": n / walk_s,
         "interactions_per_s": (newton + ewald) / walk_s,
         "interactions_per_particle": (newton + ewald) / n,
@@ -615,6 +646,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    line["stage_roofline"] = stage_roofline(head["stage"], n,
+                                            head["e"].num_nodes())
     for prec, r in res.items():
         line[prec] = summary(r)
         if r.get("tune"):

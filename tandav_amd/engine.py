@@ -450,6 +450,14 @@ class Engine:
                  "tdv_get_top_nodes")
         return buf
 
+    def num_nodes(self):
+        """Tree nodes of the last build (Tree_Node records, terminators
+        included), without copying them."""
+        n = _sz()
+        self._ok(self.lib.tdv_get_tree(self._c, None, 0, ctypes.byref(n)),
+                 "tdv_get_tree")
+        return n.value
+
     def tree(self):
         n = _sz()
         self._ok(self.lib.tdv_get_tree(self._c, None, 0, ctypes.byref(n)),
