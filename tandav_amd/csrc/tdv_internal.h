@@ -173,7 +173,7 @@ void tdv_graph_drop(tdv_context *c);
                                             start later and find the queue
                                             drained */
 #define TDV_SPLIT_ITEMS 16384            /* work items a split walk aims at */
-#define TDV_LONG_CHAIN 2048              /* build: longer chain scans go to
+#define TDV_LONG_CHAIN 256               /* build: longer chain scans go to
                                             one wavefront each */
 #define TDV_LONG_GRID 2048               /* waves of that kernel */
 #define TDV_SPLIT_AUTO_MAX 64            /* parts per group, split 0 */
