@@ -404,18 +404,47 @@ __global__ void __launch_bounds__(64) k_tree_nodes_long(
 		const int comp = lane & 3;
 		double acc = 0;
 		int cur = hi, runmin = 99;
+		/* a block's particles and keys are loaded while the previous
+		 * block is summed */
+		double nm = 0, nx = 0, ny = 0, nz = 0;
+		Key2 nk0 = { 0, 0 }, nk1 = { 0, 0 };
+		{
+			const int k = i + lane;
+			if (k <= e) {
+				nm = m[k];
+				nx = x[k];
+				ny = y[k];
+				nz = z[k];
+				nk0 = load_key(key, k);
+				if (k + 1 <= r.last)
+					nk1 = load_key(key, k + 1);
+			}
+		}
 		for (int b0 = i; b0 <= e && cur >= lo; b0 += 64) {
 			const int k = b0 + lane;
 			const bool v = k <= e;
+			const double cm = nm, cx = nx, cy = ny, cz = nz;
+			const Key2 ck0 = nk0, ck1 = nk1;
+			{
+				const int kn = k + 64;
+				if (kn <= e) {
+					nm = m[kn];
+					nx = x[kn];
+					ny = y[kn];
+					nz = z[kn];
+					nk0 = load_key(key, kn);
+					if (kn + 1 <= r.last)
+						nk1 = load_key(key, kn + 1);
+				}
+			}
 			double tx = 0, ty = 0, tz = 0, mk = 0;
 			int dn = 99;
 			if (v) {
-				mk = m[k];
-				tx = x[k] * mk;
-				ty = y[k] * mk;
-				tz = z[k] * mk;
-				dn = k + 1 <= r.last ? div_level(load_key(key, k),
-						load_key(key, k + 1)) : -1;
+				mk = cm;
+				tx = cx * mk;
+				ty = cy * mk;
+				tz = cz * mk;
+				dn = k + 1 <= r.last ? div_level(ck0, ck1) : -1;
 			}
 			/* runmin before particle k: min of the levels of i .. k-1 */
 			int incl = dn;
