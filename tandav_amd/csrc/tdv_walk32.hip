@@ -29,6 +29,7 @@
  */
 #include "tdv_internal.h"
 #include "tdv_device.h"
+#include <algorithm>
 
 struct Walk32Params {
 	int ntop, vs, periodic, use_bh;
@@ -968,6 +969,42 @@ __global__ void __launch_bounds__(64) k_walk32_queue(const int *__restrict__ nac
 	}
 }
 
+/* The same work items (group, part) with one workgroup each, the active
+ * count read from device memory: the hardware's workgroup dispatch is the
+ * queue.  A grid sized for the largest case serves any active count (a
+ * captured graph), the workgroups past the last item leave at once; with
+ * the register budgets of the plain walk. */
+#define TDV_WALK32_ITEMS_BODY                                               \
+	nact = *nact_dev;                                                      \
+	const int ngroups = (nact + 63) / 64;                                  \
+	const int nparts = walk32_parts(ngroups, split);                       \
+	const int item = blockIdx.x;                                           \
+	if (item >= ngroups * nparts)                                          \
+		return;                                                            \
+	const int g = item / nparts;                                           \
+	TDV_WALK32_CALL_PART(g, item - g * nparts, nparts, partial, ngroups * 64)
+
+__global__ void __launch_bounds__(64) k_walk32_items(
+		const int *__restrict__ nact_dev, int split,
+		double4 *__restrict__ partial, TDV_WALK32_ARGS)
+{
+	TDV_WALK32_ITEMS_BODY;
+}
+
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
+k_walk32_items_w6(const int *__restrict__ nact_dev, int split,
+		double4 *__restrict__ partial, TDV_WALK32_ARGS)
+{
+	TDV_WALK32_ITEMS_BODY;
+}
+
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8)))
+k_walk32_items_w8(const int *__restrict__ nact_dev, int split,
+		double4 *__restrict__ partial, TDV_WALK32_ARGS)
+{
+	TDV_WALK32_ITEMS_BODY;
+}
+
 /* The parts' sums of a split walk, in part order, to the particles (as
  * walk32_body writes them when it does not split) */
 __global__ void k_walk32_parts_sum(const int *__restrict__ nact_dev, int split,
@@ -1204,7 +1241,31 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	w.quad = quad ? 1 : 0;
 	w.crit3 = c->par.fp32_criterion == 3;
 	const int split = c->par.fp32_split;
-	const bool parts = walk32_parts((int)((c->nactive + 63) / 64), split) > 1;
+	const int ngroups_h = (int)((c->nactive + 63) / 64);
+	const int nparts_h = walk32_parts(ngroups_h, split);
+	const bool parts = nparts_h > 1;
+	if (!c->par.fp32_queue && (c->capturing || parts)) {
+		/* one workgroup per item; captured: enough for any active set */
+		const size_t items = c->capturing
+				? std::max((n + 63) / 64, (size_t)TDV_SPLIT_ITEMS)
+				: (size_t)ngroups_h * nparts_h;
+		auto kern = c->par.fp32_waves == 6 ? k_walk32_items_w6
+				: c->par.fp32_waves == 8 ? k_walk32_items_w8 : k_walk32_items;
+		hipLaunchKernelGGL(k_walk32_queue_reset, dim3(1), dim3(64), 0,
+				c->stream, c->d_queue, c->d_count);
+		hipLaunchKernelGGL(kern, dim3((unsigned)items), dim3(64), 0,
+				c->stream, c->d_nactive, split, c->partial, 0, c->active,
+				c->p32, c->last_acc, c->top, c->a32, c->b32, c->node_c,
+				c->ewald16, w, c->acc[0], c->acc[1], c->acc[2], c->gacc[0],
+				c->gacc[1], c->gacc[2], c->cost, c->d_count, c->top32, c->q32,
+				c->tq32, c->p32o);
+		TDV_TRY(tdv_check(c, hipGetLastError(), "k_walk32_items"));
+		hipLaunchKernelGGL(k_walk32_parts_sum, dim3(tdv_blocks(n, 256)),
+				dim3(256), 0, c->stream, c->d_nactive, split, c->active,
+				c->partial, c->acc[0], c->acc[1], c->acc[2], c->gacc[0],
+				c->gacc[1], c->gacc[2], c->cost);
+		return tdv_check(c, hipGetLastError(), "k_walk32_parts_sum");
+	}
 	if (c->par.fp32_queue || c->capturing || parts) {
 		/* grid: enough waves to fill every SIMD at the 8-wave budget */
 		hipLaunchKernelGGL(k_walk32_queue_reset, dim3(1), dim3(64), 0,
