@@ -220,6 +220,21 @@ C4 = dict(n=1 << 23, nhalo=256, seed=11, alpha=0.003,
           "20% uniform background, 2^23 = 8,388,608 particles, 8 time bins")
 
 
+def walk32_kernel(waves, queue, split, nactive):
+    """Name of the kernel an eager FP32 walk over nactive particles runs:
+    the work-queue kernel, the one-workgroup-per-item kernel when the walk
+    splits (parts per group as walk32_parts: up to 16384 items, at most 64
+    parts or `split`), else the plain grid; with the register budget."""
+    ngroups = (nactive + 63) // 64
+    kmax = split if split >= 2 else 64
+    parts = 1 if split == 1 or ngroups <= 0 else max(1, min(16384 // ngroups,
+                                                             kmax))
+    if queue:
+        return "k_walk32_queue"
+    base = "k_walk32_items" if parts > 1 else "k_walk32"
+    return base + {0: "", 6: "_w6", 8: "_w8"}[waves]
+
+
 def time_bins(e, n):
     """Time bins from |a| (the shortest steps for the largest
     accelerations): the fastest 0.05% in bin 0, fractions growing
@@ -315,7 +330,10 @@ def bench_c4(args, f32):
         "config": {"workload": C4["desc"] if n == C4["n"] else
                    C4["desc"].replace("2^23 = 8,388,608", str(n)),
                    "n_particles": n, "softening": soft,
-                   "fp32": dict(f32, fp32_queue=1),
+                   # the schedule the timed walks ran (the tuner's pick)
+                   "fp32": dict(f32, fp32_queue=int(e.params.fp32_queue),
+                                fp32_waves=int(e.params.fp32_waves),
+                                fp32_split=int(e.params.fp32_split)),
                    "error_sample_exact": len(xidx),
                    "parallelism": "single GPU"},
         "full_step": {"stages_ms": {k: round(v, 4) for k, v in stage.items()},
@@ -579,9 +597,8 @@ def main():
     if "fp32" in res:
         # the kernel the timed walk ran (the autotuner's pick, full set)
         pe = head["e"].params
-        kern = ("k_walk32_queue" if pe.fp32_queue else
-                {0: "k_walk32", 6: "k_walk32_w6",
-                 8: "k_walk32_w8"}[pe.fp32_waves])
+        kern = walk32_kernel(pe.fp32_waves, pe.fp32_queue, pe.fp32_split,
+                             head["e"].n)
         t_k = head["stage"]["walk"]
         flops = (head["newton"] * FLOPS_PER_INTERACTION
                  + head["ewald"] * FLOPS_PER_EWALD)

@@ -148,7 +148,7 @@ int tdv_tree_acceleration(tdv_context *ctx, int use_bh);
 int tdv_compute_acceleration(tdv_context *ctx, int prepare_step);
 
 /* A regular step's FP32 walk (safe_last_accel, accel.c:28-43, the walk
- * records and the work-queue walk over the current active list) captured as
+ * records and the walk over the active count held on the device) captured as
  * a HIP graph on the context's stream, for replay on substeps that reuse
  * the tree (hierarchical time steps: Active_Particle_List changes,
  * positions drift through tdv_update_particles, the tree stays).  Needs one

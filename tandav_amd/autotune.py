@@ -6,8 +6,8 @@ pick per workload is the schedule:
 
 * ``waves``: the register budget, 0 (the compiler's), 6 or 8 wavefronts per
   SIMD (``fp32_waves``);
-* ``queue``: one launch of a wavefront per group (0) or the persistent
-  work-queue kernel (1);
+* ``queue``: one wavefront launched per group, or per (group, part) work
+  item when the walk splits (0), or the persistent work-queue kernel (1);
 * ``split``: for substeps with few active groups, how many parts each
   group's subtree walks may be cut into (1 = never; n >= 2 = up to n).
 
