@@ -118,6 +118,24 @@ int tdv_setup_domain(tdv_context *ctx);
  * active list (Make_Active_Particle_List, timestep.c:280-299). */
 int tdv_domain_decomposition(tdv_context *ctx);
 
+/* Domain_Decomposition (domain.c:67-145) with NTarget = ntarget (1..512;
+ * Setup_Domain_Decomposition, domain.c:151-156, takes NTask or NRank): the
+ * domain, PH keys, key sort and reorder as tdv_domain_decomposition, then
+ * the bunch refinement loop (reset_bunchlist, fill_new_bunches on the
+ * particles' Cost, distribute, find_global_imbalances,
+ * mark_bunches_to_split, make_new_bunchlist; domain.c:229-313, 370-692)
+ * until the cost imbalance is below imbal_ceil (DOMAIN_IMBAL_CEIL, Config:11,
+ * 0.05) and the particle imbalance below part_alloc_factor - 1
+ * (PartAllocFactor, default 1.1), or after 20 passes.  The bunches become
+ * the top nodes (transform_bunches_into_top_nodes, domain.c:220-250).
+ * tasks (may be NULL, max_tasks entries): the target each top node went
+ * to, in top-node order; ntop, iterations (may be NULL): the top-node count
+ * and the loop's pass count.  Fails when fewer non-empty bunches than
+ * targets remain (the reference indexes past its bunch list there). */
+int tdv_domain_decomposition_targets(tdv_context *ctx, int ntarget,
+		double part_alloc_factor, double imbal_ceil, int *tasks, int max_tasks,
+		int *ntop, int *iterations);
+
 /* The Domain (domain.h:58-63) the caller computed, for the next
  * tdv_sort_particles. */
 int tdv_set_domain(tdv_context *ctx, double size, const double origin[3]);

@@ -124,6 +124,13 @@ def _declare(L):
     L.orc_sim_epsilon.argtypes = [_vp]
     L.orc_sim_epsilon.restype = ctypes.c_double
     L.orc_direct_sum.argtypes = [_vp, ctypes.c_int, _ip, _dp]
+    L.orc_domain_targets.argtypes = [_vp, _dp, ctypes.c_size_t, ctypes.c_int,
+                                     ctypes.c_double, ctypes.c_double, _vp,
+                                     ctypes.c_int, _ip, _ip]
+    L.orc_domain_targets.restype = ctypes.c_int
+    L.orc_sim_domain_decomposition_targets.argtypes = [
+        _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _ip, _ip]
+    L.orc_sim_domain_decomposition_targets.restype = ctypes.c_int
 
 
 def _d(a):
@@ -166,6 +173,26 @@ def domain_nonperiodic(x, y, z):
     lib().orc_domain_nonperiodic(x.size, _d(x), _d(y), _d(z), _d(c), _d(s),
                                  _d(o))
     return c, float(s[0]), o
+
+
+def domain_targets(short_keys, cost, ntarget, part_alloc_factor=1.1,
+                   imbal_ceil=0.05):
+    """Domain_Decomposition with NTarget = ntarget (domain.c:67-145):
+    short keys (upper 64 bits, ascending) and P.Cost in -> top-node records
+    in descending key order (Target = -task - 1, Pos 0) and the loop's
+    iteration count.  Raises on the oracle's error returns."""
+    k = np.ascontiguousarray(short_keys, dtype=np.uint64)
+    c = np.ascontiguousarray(cost, dtype=np.float64)
+    out = np.zeros(65536, dtype=TOP_DTYPE)
+    nt = ctypes.c_int(0)
+    it = ctypes.c_int(0)
+    rc = lib().orc_domain_targets(k.ctypes.data_as(_vp), _d(c), k.size,
+                                  int(ntarget), float(part_alloc_factor),
+                                  float(imbal_ceil), out.ctypes.data_as(_vp),
+                                  out.size, ctypes.byref(nt), ctypes.byref(it))
+    if rc != 0:
+        raise ValueError("orc_domain_targets returned %d" % rc)
+    return out[:nt.value].copy(), it.value
 
 
 def ewald_table(nthreads=None):
@@ -249,6 +276,19 @@ class Sim:
 
     def domain_decomposition(self):
         lib().orc_sim_domain_decomposition(self._s)
+
+    def domain_decomposition_targets(self, ntarget, part_alloc_factor=1.1,
+                                     imbal_ceil=0.05):
+        """Domain_Decomposition with NTarget = ntarget; returns (tasks per
+        top node, loop passes)."""
+        tasks = np.zeros(65536, dtype=np.int32)
+        it = ctypes.c_int(0)
+        rc = lib().orc_sim_domain_decomposition_targets(
+            self._s, int(ntarget), float(part_alloc_factor),
+            float(imbal_ceil), tasks.ctypes.data_as(_ip), ctypes.byref(it))
+        if rc != 0:
+            raise ValueError("orc_sim_domain_decomposition_targets: %d" % rc)
+        return tasks[:lib().orc_sim_ntop(self._s)].copy(), it.value
 
     def tree_build(self):
         lib().orc_sim_tree_build(self._s)

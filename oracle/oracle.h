@@ -139,6 +139,21 @@ const int *orc_sim_active(const orc_sim *s);
 void orc_sim_setup_domain(orc_sim *s);          /* Setup_Domain_Decomposition */
 void orc_sim_domain_decomposition(orc_sim *s);  /* Domain_Decomposition       */
 void orc_sim_set_top_nodes(orc_sim *s, const orc_topnode *rec, int ntop);
+
+/* Domain_Decomposition with NTarget = ntarget (domain.c:67-145) on
+ * particles whose short keys (upper 64 bits) are sorted ascending, with
+ * their P.Cost: the bunch list in descending key order as top-node records
+ * (Key, Target = -task - 1, Level, First_Part, Npart; Pos left 0).
+ * Returns 0, -1 on a bad argument or more than max_out bunches, -2 when
+ * fewer non-empty bunches than targets remain. */
+int orc_domain_targets(const uint64_t *key, const double *cost, size_t n,
+		int ntarget, double part_alloc_factor, double imbal_ceil,
+		orc_topnode *out, int max_out, int *ntop, int *iterations);
+/* The same on a simulation (set_computational_domain, the key sort, the
+ * loop on its P.Cost, the bunches as top nodes); tasks: >= 65536 ints. */
+int orc_sim_domain_decomposition_targets(orc_sim *s, int ntarget,
+		double part_alloc_factor, double imbal_ceil, int *tasks,
+		int *iterations);
 void orc_sim_tree_build(orc_sim *s);            /* Gravity_Tree_Build         */
 void orc_sim_tree_acceleration(orc_sim *s, int use_bh); /* + periodic part */
 void orc_sim_compute_acceleration(orc_sim *s, int prepare_step);

@@ -352,6 +352,35 @@ void orc_sim_domain_decomposition(orc_sim *s)
 			s->P.pos[2], s->dom_origin, s->dom_size, s->P.key);
 }
 
+/* Domain_Decomposition with NTarget = ntarget (domain.c:67-145): the bunch
+ * loop (orc_domain.c) on the sorted particles' short keys and P.Cost, its
+ * bunches as top nodes; tasks[i] = the target of top node i */
+int orc_sim_domain_decomposition_targets(orc_sim *s, int ntarget,
+		double part_alloc_factor, double imbal_ceil, int *tasks,
+		int *iterations)
+{
+	set_computational_domain(s);
+	sort_particles_by_peano_key(s);
+	uint64_t *k = malloc(s->npart * sizeof(*k));
+	orc_topnode *rec = malloc(65536 * sizeof(*rec));
+	for (size_t i = 0; i < s->npart; i++)
+		k[i] = (uint64_t)(s->P.key[i] >> 64);
+	int ntop = 0;
+	int rc = orc_domain_targets(k, s->P.cost, s->npart, ntarget,
+			part_alloc_factor, imbal_ceil, rec, 65536, &ntop, iterations);
+	if (rc == 0) {
+		for (int i = 0; i < ntop; i++)
+			tasks[i] = -(rec[i].Target + 1);
+		orc_sim_set_top_nodes(s, rec, ntop);
+		sort_particles_by_peano_key(s);   /* identity for unique keys */
+		orc_reversed_peano_keys(s->npart, s->P.pos[0], s->P.pos[1],
+				s->P.pos[2], s->dom_origin, s->dom_size, s->P.key);
+	}
+	free(k);
+	free(rec);
+	return rc;
+}
+
 /* ------------------------------------------------------------------------ */
 /* tree build (Gravity/tree_build.c:47-572)                                  */
 /* ------------------------------------------------------------------------ */

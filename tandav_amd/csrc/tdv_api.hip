@@ -23,11 +23,6 @@ struct NodeRecord {
 static_assert(sizeof(NodeRecord) == 96, "Tree_Node is 96 bytes");
 
 /* Top_Tree_Node layout, domain.h:40-52 (records in and out) */
-struct TopRecordIn {
-	uint64_t Key;
-	int Target, Level, First_Part, Npart;
-	float Pos[3], Mass, CoM[3], Dp[3];
-};
 
 
 int tdv_fail(tdv_context *c, int code, const std::string &msg)
@@ -671,6 +666,49 @@ int tdv_domain_decomposition(tdv_context *c)
 	c->stage_ms[2] = ms;
 	(void)hipEventElapsedTime(&ms, c->ev[EV_REORDER1], c->ev[EV_TOP1]);
 	c->stage_ms[3] = ms;
+	return 0;
+}
+
+int tdv_domain_decomposition_targets(tdv_context *c, int ntarget,
+		double part_alloc_factor, double imbal_ceil, int *tasks, int max_tasks,
+		int *ntop, int *iterations)
+{
+	if (!c)
+		return TDV_ERR_ARG;
+	if (ntarget < 1 || ntarget > 512 || !(part_alloc_factor >= 1.0)
+			|| !(imbal_ceil > 0.0) || (tasks && max_tasks < 0))
+		return tdv_fail(c, TDV_ERR_ARG,
+				"tdv_domain_decomposition_targets: bad arguments");
+	TDV_TRY(computational_domain(c));
+	c->have_domain_set = 1;
+	TDV_TRY(keys_sort_reorder(c));
+	std::vector<TopRecordIn> rec;
+	int it = 0;
+	TDV_TRY(tdv_domain_refine(c, ntarget, part_alloc_factor, imbal_ceil, rec,
+			&it));
+	TDV_TRY(tdv_set_top_nodes(c, rec.data(), (int)rec.size()));
+	(void)hipEventRecord(c->ev[EV_TOP1], c->stream);
+	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	float ms;
+	(void)hipEventElapsedTime(&ms, c->ev[EV_KEYS0], c->ev[EV_KEYS1]);
+	c->stage_ms[0] = ms;
+	(void)hipEventElapsedTime(&ms, c->ev[EV_KEYS1], c->ev[EV_SORT1]);
+	c->stage_ms[1] = ms;
+	(void)hipEventElapsedTime(&ms, c->ev[EV_SORT1], c->ev[EV_REORDER1]);
+	c->stage_ms[2] = ms;
+	(void)hipEventElapsedTime(&ms, c->ev[EV_REORDER1], c->ev[EV_TOP1]);
+	c->stage_ms[3] = ms;
+	if (ntop)
+		*ntop = (int)rec.size();
+	if (iterations)
+		*iterations = it;
+	if (tasks) {
+		if ((int)rec.size() > max_tasks)
+			return tdv_fail(c, TDV_ERR_ARG, "tdv_domain_decomposition_targets:"
+					" tasks array too small");
+		for (size_t i = 0; i < rec.size(); i++)
+			tasks[i] = -(rec[i].Target + 1);
+	}
 	return 0;
 }
 

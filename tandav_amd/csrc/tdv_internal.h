@@ -19,6 +19,7 @@
 #include <stdint.h>
 #include <stddef.h>
 #include <string>
+#include <vector>
 
 #include "../../include/tandav_gpu.h"
 
@@ -186,6 +187,19 @@ int tdv_ewald_derivs_device(tdv_context *c, double *out10, double *out6);
 int tdv_ewald_derivs_ready(tdv_context *c);
 void tdv_launch_ewald_correction(tdv_context *c, size_t n, const double *dr,
 		double *f);
+
+/* D[].TNode (domain.h:40-52) as the C-ABI passes it: 64-byte records */
+struct TopRecordIn {
+	uint64_t Key;
+	int Target, Level, First_Part, Npart;
+	float Pos[3], Mass, CoM[3], Dp[3];
+};
+
+/* Domain_Decomposition's bunch loop with NTarget = ntarget on the sorted
+ * particles (tdv_domain.hip): top-node records in descending key order,
+ * Target = -task - 1 */
+int tdv_domain_refine(tdv_context *c, int ntarget, double part_alloc_factor,
+		double imbal_ceil, std::vector<TopRecordIn> &out, int *iterations);
 
 /* ---- helpers (tdv_api.hip) ---- */
 int tdv_fail(tdv_context *c, int code, const std::string &msg);

@@ -80,6 +80,10 @@ def load_library(path=LIB_PATH):
     L.tdv_set_domain.argtypes = [_ctx, ctypes.c_double, _dp]
     L.tdv_sort_particles.argtypes = [_ctx]
     L.tdv_set_top_nodes.argtypes = [_ctx, _vp, ctypes.c_int]
+    _ip = ctypes.POINTER(ctypes.c_int)
+    L.tdv_domain_decomposition_targets.argtypes = [
+        _ctx, ctypes.c_int, ctypes.c_double, ctypes.c_double, _ip,
+        ctypes.c_int, _ip, _ip]
     L.tdv_tree_acceleration.argtypes = [_ctx, ctypes.c_int]
     L.tdv_compute_acceleration.argtypes = [_ctx, ctypes.c_int]
     L.tdv_forcetest.argtypes = [_ctx, ctypes.c_int, _vp, _dp]
@@ -258,6 +262,23 @@ class Engine:
         """Domain_Decomposition (domain.c:67-145)."""
         self._ok(self.lib.tdv_domain_decomposition(self._c),
                  "tdv_domain_decomposition")
+
+    def domain_decomposition_targets(self, ntarget, part_alloc_factor=1.1,
+                                     imbal_ceil=0.05):
+        """Domain_Decomposition (domain.c:67-145) with NTarget = ntarget:
+        top nodes refined until every target's share of P.Cost is within
+        imbal_ceil (DOMAIN_IMBAL_CEIL) of the mean.  Returns (tasks per top
+        node, loop passes)."""
+        tasks = np.zeros(65536, dtype=np.int32)
+        nt = ctypes.c_int(0)
+        it = ctypes.c_int(0)
+        ip = ctypes.POINTER(ctypes.c_int)
+        self._ok(self.lib.tdv_domain_decomposition_targets(
+            self._c, int(ntarget), float(part_alloc_factor),
+            float(imbal_ceil), tasks.ctypes.data_as(ip), tasks.size,
+            ctypes.byref(nt), ctypes.byref(it)),
+            "tdv_domain_decomposition_targets")
+        return tasks[:nt.value].copy(), it.value
 
     def set_domain(self, size, origin):
         """Domain (domain.h:58-63) computed by the caller."""
