@@ -358,6 +358,93 @@ C5 = dict(ngrid=512, seed=7, rms=0.5, a=0.1, dlna=0.01, omega_m=0.3,
           "a = 0.1 (flat LCDM, Omega_m 0.3, dln a 0.01)")
 
 
+def bench_domain(args, f32):
+    """Domain_Decomposition with NTarget > 1 (domain.c:67-145) on the
+    clustered box, P.Cost from one FP32 walk: the whole decomposition
+    (keys, sort, reorder, the bunch loop, top nodes) per target count, the
+    bunch loop against the oracle's on the same keys and costs (its time is
+    the CPU baseline), and the single-target decomposition beside it."""
+    import oracle
+    from tandav_amd import Engine, ics
+    n = args.c4_n or C4["n"]
+    log("domain: generating clustered box, N = %d" % n)
+    pos, _, m = ics.clustered_box(n, nhalo=C4["nhalo"], seed=C4["seed"])
+    soft = 1.0 / (40.0 * n ** (1.0 / 3.0))
+    e = Engine(periodic=True, boxsize=1.0, softening=soft, theta_bh=0.1,
+               alpha_rel=C4["alpha"], compat=False, precision="fp32",
+               ewald_size=f32["ewald_size"],
+               ewald_accuracy=f32["ewald_accuracy"],
+               fp32_order=f32["fp32_order"],
+               fp32_criterion=f32["fp32_criterion"])
+    e.set_particles(pos, m)
+    e.setup_domain()
+    e.domain_decomposition()
+    e.compute_acceleration(prepare_step=True)      # P.Cost of every particle
+    log("domain: costs from one walk")
+    e.domain_decomposition()
+    t0 = time.perf_counter()
+    e.domain_decomposition()
+    t_one = time.perf_counter() - t0
+    one = e.stage_times()
+    rows = []
+    for nt in [int(v) for v in args.ntargets.split(",")]:
+        e.domain_decomposition_targets(nt)          # warm, same input
+        t0 = time.perf_counter()
+        tasks, passes = e.domain_decomposition_targets(nt)
+        t = time.perf_counter() - t0
+        st = e.stage_times()
+        keys = e.keys(reversed_=False)[:, 1].copy()
+        cost = e.get("cost")["cost"]
+        t0 = time.perf_counter()
+        rec, passes_o = oracle.domain_targets(keys, cost, nt)
+        t_cpu = time.perf_counter() - t0
+        top = e.top_nodes()
+        same = (passes_o == passes and len(rec) == len(tasks)
+                and bool(np.array_equal(-(rec["Target"] + 1), tasks))
+                and bool(np.array_equal(rec["Key"], top["Key"]))
+                and bool(np.array_equal(rec["First_Part"], top["First_Part"])))
+        w = np.maximum(1, cost).astype(np.int64)
+        bc = np.add.reduceat(w, top["First_Part"][::-1])[::-1]
+        tc = np.bincount(tasks, weights=bc, minlength=nt)
+        rows.append({"ntarget": nt, "ms": t * 1e3,
+                     "stages_ms": {k: round(st[k], 4) for k in
+                                   ("keys", "sort", "reorder", "top_nodes")},
+                     "passes": passes, "top_nodes": len(tasks),
+                     "max_cost_imbalance": float(np.max(np.abs(tc - tc.mean())
+                                                        / tc.mean())),
+                     "oracle_loop_ms": t_cpu * 1e3, "matches_oracle": same})
+        log("domain: ntarget %d: %.2f ms (bunch loop %.2f ms), %d passes, "
+            "%d top nodes, oracle loop %.1f ms, same %s" % (
+                nt, t * 1e3, st["top_nodes"], passes, len(tasks),
+                t_cpu * 1e3, same))
+    head = next((r for r in rows if r["ntarget"] == 16), rows[-1])
+    line = {
+        "metric": "particles/s through Domain_Decomposition with NTarget "
+                  "targets (clustered box)",
+        "value": n / (head["ms"] * 1e-3), "unit": "particles/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": head["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64/f32", "data": "synthetic",
+        "config": {"workload": C4["desc"] if n == C4["n"] else
+                   C4["desc"].replace("2^23 = 8,388,608", str(n)),
+                   "n_particles": n, "ntarget": head["ntarget"],
+                   "cost": "P.Cost of one FP32 walk",
+                   "parallelism": "single GPU"},
+        "ntargets": rows,
+        "single_target": {"ms": t_one * 1e3,
+                          "stages_ms": {k: round(one[k], 4) for k in
+                                        ("keys", "sort", "reorder",
+                                         "top_nodes")}},
+        "cpu_baseline": {"value": n / (head["oracle_loop_ms"] * 1e-3),
+                         "unit": "particles/s", "cores": 1, "kind": "port",
+                         "sample": "the oracle's bunch loop (fills, "
+                                   "distribute, refinement) on the same "
+                                   "sorted keys and costs, keys and sort "
+                                   "excluded"},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_c5(args, f32):
     """One full step of the large box with every particle active, device
     resident: PH keys, radix sort, reorder, top nodes, build, the FP32 walk
@@ -474,7 +561,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2",
-                    choices=sorted(WORKLOADS) + ["c4", "c5"])
+                    choices=sorted(WORKLOADS) + ["c4", "c5", "domain"])
+    ap.add_argument("--ntargets", default="2,8,16,64",
+                    help="target counts of the domain workload")
     ap.add_argument("--c4-n", type=int, default=0,
                     help="particles of the clustered box (default 2^23)")
     ap.add_argument("--c5-ngrid", type=int, default=0,
@@ -537,6 +626,9 @@ def main():
         return
     if args.workload == "c5":
         bench_c5(args, f32)
+        return
+    if args.workload == "domain":
+        bench_domain(args, f32)
         return
 
     log("generating %s" % args.workload)

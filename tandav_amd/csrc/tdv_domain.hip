@@ -7,8 +7,8 @@
  * searches of its largest short key (and its predecessor's) over the
  * parent's range, and its Cost is the float sum of cost_metric over that
  * range in particle order (fill_new_bunches, domain.c:462-531, with one
- * thread), one lane per bunch.  The loop's bookkeeping over at most a few
- * thousand bunches (remove_empty_bunches, the key and cost sorts,
+ * thread), one wavefront per bunch.  The loop's bookkeeping over at most a
+ * few thousand bunches (remove_empty_bunches, the key and cost sorts,
  * distribute, find_global_imbalances, mark_bunches_to_split,
  * make_new_bunchlist; domain.c:370-460, 571-676) runs on the host, in the
  * reference's types (float bunch and task costs, double means).  The sorts
@@ -66,31 +66,61 @@ __device__ __forceinline__ long long upper_bound_short(
 	return lo;
 }
 
-/* fill_new_bunches: a particle joins the first bunch (ascending keys)
- * whose key is not below its own; cost_metric = (unsigned) fmax(1, Cost)
- * summed in float, one particle after the other */
-__global__ void k_bunch_fill(int nb, const BunchFill *__restrict__ req,
-		const uint64_t *__restrict__ key, const double *__restrict__ cost,
-		BunchSum *__restrict__ out)
+/* cost_metric (domain.c:533-536): (unsigned) fmax(1, P.Cost), as a float
+ * operand of the bunch's float sum */
+__device__ __forceinline__ float cost_metric_f(const double *__restrict__ cost,
+		long long i)
 {
-	const int b = blockIdx.x * blockDim.x + threadIdx.x;
+	return (float)(unsigned)fmax(1.0, cost[i]);
+}
+
+/* fill_new_bunches: a particle joins the first bunch (ascending keys)
+ * whose key is not below its own; the bunch's Cost is the float sum of
+ * cost_metric one particle after the other.  One wavefront per bunch: the
+ * lanes load and convert 64 particles at a time (the next 64 in flight),
+ * lane 0 adds them in order from LDS. */
+__global__ void __launch_bounds__(64) k_bunch_fill(int nb,
+		const BunchFill *__restrict__ req, const uint64_t *__restrict__ key,
+		const double *__restrict__ cost, BunchSum *__restrict__ out)
+{
+	__shared__ float buf[64];
+	__shared__ long long range[2];
+	const int b = blockIdx.x;
+	const int lane = threadIdx.x;
 	if (b >= nb)
 		return;
-	const BunchFill r = req[b];
-	const long long a = r.has_prev ? upper_bound_short(key, r.lo, r.hi, r.prev)
-			: r.lo;
-	const long long e = upper_bound_short(key, a, r.hi, r.key);
-	float s = 0.f;
-	for (long long i = a; i < e; i++) {
-		const unsigned u = (unsigned)fmax(1.0, cost[i]);
-		s = __fadd_rn(s, (float)u);
+	if (lane == 0) {
+		const BunchFill r = req[b];
+		const long long a = r.has_prev
+				? upper_bound_short(key, r.lo, r.hi, r.prev) : r.lo;
+		range[0] = a;
+		range[1] = upper_bound_short(key, a, r.hi, r.key);
 	}
-	BunchSum o;
-	o.first = a < e ? (int)a : INT_MAX;
-	o.npart = (int)(e - a);
-	o.cost = s;
-	o.pad = 0;
-	out[b] = o;
+	__syncthreads();
+	const long long a = range[0], e = range[1];
+	float s = 0.f;
+	float next = a + lane < e ? cost_metric_f(cost, a + lane) : 0.f;
+	for (long long base = a; base < e; base += 64) {
+		const float v = next;
+		if (base + 64 + lane < e)
+			next = cost_metric_f(cost, base + 64 + lane);
+		buf[lane] = v;
+		__syncthreads();
+		if (lane == 0) {
+			const int cnt = (int)(e - base < 64 ? e - base : 64);
+			for (int k = 0; k < cnt; k++)
+				s = __fadd_rn(s, buf[k]);
+		}
+		__syncthreads();
+	}
+	if (lane == 0) {
+		BunchSum o;
+		o.first = a < e ? (int)a : INT_MAX;
+		o.npart = (int)(e - a);
+		o.cost = s;
+		o.pad = 0;
+		out[b] = o;
+	}
 }
 
 __global__ void k_first_positions(int nt, const int *__restrict__ first,
@@ -166,8 +196,8 @@ int fill_groups(tdv_context *c, FillScratch &fs, std::vector<Bunch> &D,
 	TDV_TRY(fs.reserve(req.size()));
 	TDV_HIP(c, hipMemcpyAsync(fs.req, req.data(), req.size() * sizeof(BunchFill),
 			hipMemcpyHostToDevice, c->stream));
-	hipLaunchKernelGGL(k_bunch_fill, dim3(tdv_blocks(req.size(), 64)), dim3(64),
-			0, c->stream, (int)req.size(), fs.req, c->key, c->cost, fs.sum);
+	hipLaunchKernelGGL(k_bunch_fill, dim3((unsigned)req.size()), dim3(64), 0,
+			c->stream, (int)req.size(), fs.req, c->key, c->cost, fs.sum);
 	TDV_HIP(c, hipGetLastError());
 	std::vector<BunchSum> sum(req.size());
 	TDV_HIP(c, hipMemcpyAsync(sum.data(), fs.sum, sum.size() * sizeof(BunchSum),
