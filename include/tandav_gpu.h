@@ -320,6 +320,52 @@ int tdv_kick_drift(tdv_context *ctx, double kick, double drift);
 /* Number of particles currently on the device. */
 size_t tdv_num_particles(const tdv_context *ctx);
 
+/* struct gadget_header (IO/io.h:23-42): the 256-byte Gadget header. */
+typedef struct {
+	uint32_t Npart[6];
+	double Massarr[6];
+	double Time;
+	double Redshift;
+	int32_t Flag_Sfr;
+	int32_t Flag_Feedback;
+	uint32_t Nall[6];
+	int32_t Flag_Cooling;
+	int32_t Num_Files;
+	double Boxsize;
+	double Omega0;
+	double Omega_Lambda;
+	double Hubble_Param;
+	int32_t Flag_Age;
+	int32_t Flag_Metals;
+	uint32_t Nall_High_Word[6];
+	char fill_bytes[64];           /* 59 in io.h + the struct's padding */
+} tdv_snapshot_header;
+
+#define TDV_BLOCK_ACC  1u          /* OUTPUT_TOTAL_ACCELERATION (Config:15)    */
+#define TDV_BLOCK_GACC 2u          /* OUTPUT_PARTIAL_ACCELERATIONS (Config:16) */
+
+/* write_file + write_gadget_header (IO/write_snapshot.c:63-219) for one
+ * rank and one file: a Gadget format-2 snapshot of the context's particles
+ * in their current order, blocks HEAD, POS, VEL, ID (uint32), MASS, then
+ * ACC and GACC when `blocks` asks for them (IO/io.h:68-89 with
+ * DOUBLE_PRECISION, no LONG_IDS).  ACC holds the x component of P.Acc: the
+ * reference's block table gives it one component (io.h:76).  head->Npart
+ * must add up to the particle count; Nall and Num_Files are set for one
+ * file and the bytes the reference leaves unset are zero. */
+int tdv_write_snapshot(tdv_context *ctx, const char *path,
+		const tdv_snapshot_header *head, unsigned blocks);
+
+/* Read_Snapshot (IO/read_snapshot.c:19-128) for one rank: the file `path`,
+ * or path.0, path.1, ... (find_files, :545-574); HEAD, then POS, VEL, ID
+ * (required) and MASS (optional, io.h:74) of each file in order; masses
+ * from the header's Massarr when any entry is nonzero
+ * (generate_masses_from_header, :421-446).  The particles go to the
+ * context as tdv_set_particles puts them; *head (may be NULL) gets the
+ * first file's header.  Files written in the other byte order are refused
+ * (the reference's swap in safe_fread, :590-607, scrambles arrays). */
+int tdv_read_snapshot(tdv_context *ctx, const char *path,
+		tdv_snapshot_header *head);
+
 #ifdef __cplusplus
 }
 #endif

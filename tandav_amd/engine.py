@@ -118,6 +118,11 @@ def load_library(path=LIB_PATH):
                                        ctypes.c_int]
     L.tdv_capture_walk.argtypes = [_ctx]
     L.tdv_launch_walk.argtypes = [_ctx]
+    L.tdv_write_snapshot.argtypes = [_ctx, ctypes.c_char_p,
+                                     ctypes.POINTER(SnapshotHeader),
+                                     ctypes.c_uint]
+    L.tdv_read_snapshot.argtypes = [_ctx, ctypes.c_char_p,
+                                    ctypes.POINTER(SnapshotHeader)]
     L.tdv_num_particles.argtypes = [_ctx]
     L.tdv_num_particles.restype = _sz
     _lib = L
@@ -126,6 +131,25 @@ def load_library(path=LIB_PATH):
 
 def _d(a):
     return a.ctypes.data_as(_dp)
+
+
+class SnapshotHeader(ctypes.Structure):
+    """struct gadget_header (IO/io.h:23-42), 256 bytes."""
+    _fields_ = [("Npart", ctypes.c_uint32 * 6),
+                ("Massarr", ctypes.c_double * 6),
+                ("Time", ctypes.c_double), ("Redshift", ctypes.c_double),
+                ("Flag_Sfr", ctypes.c_int32), ("Flag_Feedback", ctypes.c_int32),
+                ("Nall", ctypes.c_uint32 * 6),
+                ("Flag_Cooling", ctypes.c_int32), ("Num_Files", ctypes.c_int32),
+                ("Boxsize", ctypes.c_double), ("Omega0", ctypes.c_double),
+                ("Omega_Lambda", ctypes.c_double),
+                ("Hubble_Param", ctypes.c_double),
+                ("Flag_Age", ctypes.c_int32), ("Flag_Metals", ctypes.c_int32),
+                ("Nall_High_Word", ctypes.c_uint32 * 6),
+                ("fill_bytes", ctypes.c_char * 64)]
+
+
+assert ctypes.sizeof(SnapshotHeader) == 256
 
 
 def _col3(cols):
@@ -279,6 +303,44 @@ class Engine:
             ctypes.byref(nt), ctypes.byref(it)),
             "tdv_domain_decomposition_targets")
         return tasks[:nt.value].copy(), it.value
+
+    def write_snapshot(self, path, npart=None, massarr=(0.0,) * 6, time=0.0,
+                       redshift=0.0, boxsize=0.0, omega0=0.0,
+                       omega_lambda=0.0, hubble_param=0.0, acc=True,
+                       gacc=True):
+        """Write_Snapshot (IO/write_snapshot.c) for one file: HEAD, POS,
+        VEL, ID, MASS, and ACC / GACC (OUTPUT_TOTAL_ACCELERATION,
+        OUTPUT_PARTIAL_ACCELERATIONS).  npart: particles per type (default
+        all of type 1)."""
+        h = SnapshotHeader()
+        counts = [0, self.n, 0, 0, 0, 0] if npart is None else list(npart)
+        for t in range(6):
+            h.Npart[t] = int(counts[t])
+            h.Massarr[t] = float(massarr[t])
+        h.Time, h.Redshift, h.Boxsize = float(time), float(redshift), \
+            float(boxsize)
+        h.Omega0, h.Omega_Lambda, h.Hubble_Param = float(omega0), \
+            float(omega_lambda), float(hubble_param)
+        blocks = (1 if acc else 0) | (2 if gacc else 0)
+        self._ok(self.lib.tdv_write_snapshot(self._c, os.fsencode(path),
+                                             ctypes.byref(h), blocks),
+                 "tdv_write_snapshot")
+
+    def read_snapshot(self, path):
+        """Read_Snapshot (IO/read_snapshot.c) into this engine; returns the
+        (first file's) header as a dict."""
+        h = SnapshotHeader()
+        self._ok(self.lib.tdv_read_snapshot(self._c, os.fsencode(path),
+                                            ctypes.byref(h)),
+                 "tdv_read_snapshot")
+        self.n = int(self.lib.tdv_num_particles(self._c))
+        out = {}
+        for name, _ in SnapshotHeader._fields_:
+            if name == "fill_bytes":
+                continue
+            v = getattr(h, name)
+            out[name] = list(v) if hasattr(v, "__len__") else v
+        return out
 
     def set_domain(self, size, origin):
         """Domain (domain.h:58-63) computed by the caller."""
