@@ -542,7 +542,11 @@ def bench_c5(args, f32):
         "walk_force_evals_per_s": n / walk_s,
         "interactions_per_s": (newton + ewald) / walk_s,
         "interactions_per_particle": (newton + ewald) / n,
-        "walk_roofline": {"bound": "valu", "achieved": flops / walk_s / 1e12,
+        "walk_roofline": {"bound": "valu",
+                          "kernel": walk32_kernel(e.params.fp32_waves,
+                                                  e.params.fp32_queue,
+                                                  e.params.fp32_split, n),
+                          "achieved": flops / walk_s / 1e12,
                           "peak": 157.3, "unit": "TFLOP/s",
                           "frac": flops / walk_s / 1e12 / 157.3,
                           "flops_per_interaction": "20 Newton, 60 Ewald"},
