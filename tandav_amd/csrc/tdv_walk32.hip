@@ -226,45 +226,79 @@ __device__ __forceinline__ int next_sibling(const int4 *__restrict__ nc,
 	return d > 0 ? ch + d : ch + 1;
 }
 
+/* lane `sub` of a node's eight takes particles first + sub, first + sub + 8,
+ * ... of the range */
+__device__ __forceinline__ void moments_add_strided(
+		const double4 *__restrict__ pm, int first, int count, int sub,
+		double cx, double cy, double cz, double *m6)
+{
+	for (int p = first + sub; p < first + count; p += 8) {
+		const double4 q = pm[p];
+		const double dx = q.x - cx, dy = q.y - cy, dz = q.z - cz;
+		const double mx = q.w * dx, my = q.w * dy;
+		m6[0] = fma(mx, dx, m6[0]);
+		m6[1] = fma(my, dy, m6[1]);
+		m6[2] = fma(q.w * dz, dz, m6[2]);
+		m6[3] = fma(mx, dy, m6[3]);
+		m6[4] = fma(mx, dz, m6[4]);
+		m6[5] = fma(my, dz, m6[5]);
+	}
+}
+
+/* Eight lanes per node: each takes every eighth particle of the ranges no
+ * child covers and every eighth child term, then the eight partial sums
+ * meet in a fixed butterfly (the same order on every run; every lane ends
+ * with the same sums, addition being commutative). */
 __global__ void __launch_bounds__(256) k_moments_level(int count,
 		const int *__restrict__ list, const int4 *__restrict__ nc,
 		const double4 *__restrict__ na, const double4 *__restrict__ pm,
 		double *__restrict__ s6, double scale, float4 *__restrict__ q)
 {
-	const int t = blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= count)
-		return;
-	const int j = list[t];
-	const int4 c = nc[j];
-	const double4 a = na[j];
+	const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+	const int t = gt >> 3, sub = gt & 7;
+	const bool on = t < count;          /* the same for a node's 8 lanes */
 	double m6[6] = { 0, 0, 0, 0, 0, 0 };
-	if (c.x > 1) {
-		int covered = c.w;
-		const int end = j + c.x;
-		for (int ch = j + 1; ch < end; ch = next_sibling(nc, ch)) {
-			const int4 cc = nc[ch];
-			moments_add_particles(pm, covered, cc.w - covered, a.x, a.y, a.z,
-					m6);
-			const double4 ca = na[ch];
-			const double dx = ca.x - a.x, dy = ca.y - a.y, dz = ca.z - a.z;
-			const double *sc = s6 + 6 * (size_t)ch;
-			const double mx = ca.w * dx, my = ca.w * dy;
-			m6[0] += sc[0] + mx * dx;
-			m6[1] += sc[1] + my * dy;
-			m6[2] += sc[2] + ca.w * dz * dz;
-			m6[3] += sc[3] + mx * dy;
-			m6[4] += sc[4] + mx * dz;
-			m6[5] += sc[5] + my * dz;
-			covered = cc.w + cc.y;
+	int j = 0;
+	if (on) {
+		j = list[t];
+		const int4 c = nc[j];
+		const double4 a = na[j];
+		if (c.x > 1) {
+			int covered = c.w, k = 0;
+			const int end = j + c.x;
+			for (int ch = j + 1; ch < end; ch = next_sibling(nc, ch), k++) {
+				const int4 cc = nc[ch];
+				moments_add_strided(pm, covered, cc.w - covered, sub, a.x, a.y,
+						a.z, m6);
+				if ((k & 7) == sub) {
+					const double4 ca = na[ch];
+					const double dx = ca.x - a.x, dy = ca.y - a.y,
+							dz = ca.z - a.z;
+					const double *sc = s6 + 6 * (size_t)ch;
+					const double mx = ca.w * dx, my = ca.w * dy;
+					m6[0] += sc[0] + mx * dx;
+					m6[1] += sc[1] + my * dy;
+					m6[2] += sc[2] + ca.w * dz * dz;
+					m6[3] += sc[3] + mx * dy;
+					m6[4] += sc[4] + mx * dz;
+					m6[5] += sc[5] + my * dz;
+				}
+				covered = cc.w + cc.y;
+			}
+			moments_add_strided(pm, covered, c.w + c.y - covered, sub, a.x,
+					a.y, a.z, m6);
+		} else {
+			moments_add_strided(pm, c.w, c.y, sub, a.x, a.y, a.z, m6);
 		}
-		moments_add_particles(pm, covered, c.w + c.y - covered, a.x, a.y,
-				a.z, m6);
-	} else {
-		moments_add_particles(pm, c.w, c.y, a.x, a.y, a.z, m6);
 	}
 	for (int k = 0; k < 6; k++)
-		s6[6 * (size_t)j + k] = m6[k];
-	moments_store(m6, scale, q + 2 * j);
+		for (int d = 4; d >= 1; d >>= 1)
+			m6[k] += __shfl_xor(m6[k], d, 8);
+	if (on && sub == 0) {
+		for (int k = 0; k < 6; k++)
+			s6[6 * (size_t)j + k] = m6[k];
+		moments_store(m6, scale, q + 2 * j);
+	}
 }
 
 /* per-level node lists (terminators left out); counts are gathered per
@@ -1170,7 +1204,7 @@ int tdv_walk32(tdv_context *c, int use_bh)
 		for (int lv = 63; nn && lv >= 0; lv--)
 			if (c->lvl_cnt[lv] > 0)
 				hipLaunchKernelGGL(k_moments_level,
-						dim3(tdv_blocks((size_t)c->lvl_cnt[lv], 256)),
+						dim3(tdv_blocks(8 * (size_t)c->lvl_cnt[lv], 256)),
 						dim3(256), 0, c->stream, c->lvl_cnt[lv],
 						c->lvl_nodes + c->lvl_off[lv], c->node_c, c->node_a,
 						c->pm, c->q_s6, qscale, c->q32);
