@@ -72,17 +72,6 @@ __device__ __forceinline__ unsigned fixp(double x, double o, double S)
 	return (unsigned)(unsigned long long)u;
 }
 
-__global__ void k_prep32_particles(size_t n, const double4 *__restrict__ pm,
-		double o0, double o1, double o2, double S, double mscale,
-		int4 *__restrict__ p32)
-{
-	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n)
-		return;
-	const double4 p = pm[i];
-	p32[i] = make_int4((int)fixp(p.x, o0, S), (int)fixp(p.y, o1, S),
-			(int)fixp(p.z, o2, S), __float_as_int((float)(p.w * mscale)));
-}
 
 __global__ void k_prep32_nodes(size_t nn, const double4 *__restrict__ na,
 		const double4 *__restrict__ nb, double o0, double o1, double o2,
@@ -153,19 +142,24 @@ __global__ void k_prep32_ewald_taylor(const double *__restrict__ raw,
 			0.f);
 }
 
-/* Every particle of a bundle: its offset from the bundle's centre of mass
- * in fixed units (an exact integer difference, rounded once to FP32) and
+/* Every particle's fixed-point record (position, mass') and, in one pass
+ * (after the nodes' records), its offset from its bundle's centre of mass
+ * in fixed units (an exact integer difference, rounded once to FP32) with
  * its mass', so a group taking the bundle in the centre's frame adds one
  * FP32 add per component.  Particles of small top nodes (no leaf) are not
- * read from here. */
-__global__ void k_prep32_offsets(size_t n, const int4 *__restrict__ p32,
+ * read from the offsets. */
+__global__ void k_prep32_particles(size_t n, const double4 *__restrict__ pm,
+		double o0, double o1, double o2, double S, double mscale,
 		const int *__restrict__ leaf, const int4 *__restrict__ a32,
-		float4 *__restrict__ p32o)
+		int4 *__restrict__ p32, float4 *__restrict__ p32o)
 {
 	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
-	const int4 p = p32[i];
+	const double4 q = pm[i];
+	const int4 p = make_int4((int)fixp(q.x, o0, S), (int)fixp(q.y, o1, S),
+			(int)fixp(q.z, o2, S), __float_as_int((float)(q.w * mscale)));
+	p32[i] = p;
 	const int j = leaf[i];
 	float4 o = make_float4(0.f, 0.f, 0.f, __int_as_float(p.w));
 	if (j >= 0) {
@@ -1153,8 +1147,6 @@ int tdv_walk32(tdv_context *c, int use_bh)
 		c->n32_cap = c->node_cap;
 	}
 	const bool quad = c->par.fp32_order != 1;
-	hipLaunchKernelGGL(k_prep32_particles, dim3(tdv_blocks(n, 256)), dim3(256),
-			0, c->stream, n, c->pm, o[0], o[1], o[2], S, mscale, c->p32);
 	if (c->top32_cap < c->top_cap) {
 		tdv_graph_drop(c);
 		if (c->top32)
@@ -1219,8 +1211,9 @@ int tdv_walk32(tdv_context *c, int use_bh)
 		hipLaunchKernelGGL(k_prep32_nodes, dim3(tdv_blocks(nn, 256)),
 				dim3(256), 0, c->stream, nn, c->node_a, c->node_b, o[0], o[1],
 				o[2], S, mscale, c->a32, c->b32);
-	hipLaunchKernelGGL(k_prep32_offsets, dim3(tdv_blocks(n, 256)), dim3(256),
-			0, c->stream, n, c->p32, c->tree_parent, c->a32, c->p32o);
+	hipLaunchKernelGGL(k_prep32_particles, dim3(tdv_blocks(n, 256)), dim3(256),
+			0, c->stream, n, c->pm, o[0], o[1], o[2], S, mscale,
+			c->tree_parent, c->a32, c->p32, c->p32o);
 	if (c->par.periodic && !c->ewald16_ok) {
 		if (!c->have_ewald)
 			return tdv_fail(c, TDV_ERR_ARG, "FP32 walk without an Ewald table");
