@@ -66,24 +66,21 @@ __device__ __forceinline__ long long upper_bound_short(
 	return lo;
 }
 
-/* cost_metric (domain.c:533-536): (unsigned) fmax(1, P.Cost), as a float
- * operand of the bunch's float sum */
-__device__ __forceinline__ float cost_metric_f(const double *__restrict__ cost,
-		long long i)
-{
-	return (float)(unsigned)fmax(1.0, cost[i]);
-}
+#define TDV_FILL_PER_LANE 16              /* particles per lane per batch */
 
 /* fill_new_bunches: a particle joins the first bunch (ascending keys)
  * whose key is not below its own; the bunch's Cost is the float sum of
- * cost_metric one particle after the other.  One wavefront per bunch: the
- * lanes load and convert 64 particles at a time (the next 64 in flight),
- * lane 0 adds them in order from LDS. */
+ * cost_metric (domain.c:533-536: (unsigned) fmax(1, P.Cost)) one particle
+ * after the other.  One wavefront per bunch, in
+ * batches of 64 x TDV_FILL_PER_LANE particles: the lanes convert a batch
+ * (loaded during the previous one) into LDS, issue the next batch's loads
+ * and lane 0 adds the batch in particle order while they are in flight. */
 __global__ void __launch_bounds__(64) k_bunch_fill(int nb,
 		const BunchFill *__restrict__ req, const uint64_t *__restrict__ key,
 		const double *__restrict__ cost, BunchSum *__restrict__ out)
 {
-	__shared__ float buf[64];
+	constexpr int B = TDV_FILL_PER_LANE;
+	__shared__ float buf[64 * B];
 	__shared__ long long range[2];
 	const int b = blockIdx.x;
 	const int lane = threadIdx.x;
@@ -99,19 +96,36 @@ __global__ void __launch_bounds__(64) k_bunch_fill(int nb,
 	__syncthreads();
 	const long long a = range[0], e = range[1];
 	float s = 0.f;
-	float next = a + lane < e ? cost_metric_f(cost, a + lane) : 0.f;
-	for (long long base = a; base < e; base += 64) {
-		const float v = next;
-		if (base + 64 + lane < e)
-			next = cost_metric_f(cost, base + 64 + lane);
-		buf[lane] = v;
-		__syncthreads();
-		if (lane == 0) {
-			const int cnt = (int)(e - base < 64 ? e - base : 64);
-			for (int k = 0; k < cnt; k++)
-				s = __fadd_rn(s, buf[k]);
+	if (a < e) {
+		/* loads at indices clamped into [a, e): every batch issues the same
+		 * loads; particles past e are not added */
+		double raw[B];
+#pragma unroll
+		for (int u = 0; u < B; u++) {
+			const long long k = a + lane + 64 * u;
+			raw[u] = cost[k < e ? k : e - 1];
 		}
-		__syncthreads();
+		for (long long base = a; base < e; base += 64 * B) {
+			float v[B];
+#pragma unroll
+			for (int u = 0; u < B; u++)
+				v[u] = (float)(unsigned)fmax(1.0, raw[u]);
+#pragma unroll
+			for (int u = 0; u < B; u++) {
+				const long long k = base + 64 * B + lane + 64 * u;
+				raw[u] = cost[k < e ? k : e - 1];
+			}
+#pragma unroll
+			for (int u = 0; u < B; u++)
+				buf[lane + 64 * u] = v[u];
+			__syncthreads();
+			if (lane == 0) {
+				const int cnt = (int)(e - base < 64 * B ? e - base : 64 * B);
+				for (int q = 0; q < cnt; q++)
+					s = __fadd_rn(s, buf[q]);
+			}
+			__syncthreads();
+		}
 	}
 	if (lane == 0) {
 		BunchSum o;
