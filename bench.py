@@ -256,8 +256,8 @@ def bench_c4(args, f32):
     particle active), then one substep per bin level with the active set of
     bins <= b (Make_Active_Particle_List, timestep.c:280-299), the tree
     kept.  Each substep runs the FP32 walk eagerly and as a replay of the
-    captured walk graph (safe_last_accel + walk records + the persistent
-    work-queue walk, one launch for any active set)."""
+    captured walk graph (safe_last_accel + walk records + the walk over
+    the active count held on the device, one launch for any active set)."""
     from tandav_amd import Engine, ics
     n = args.c4_n or C4["n"]
     log("generating clustered box, N = %d" % n)

@@ -74,7 +74,9 @@ typedef struct {
 	int fp32_waves;        /* FP32 walk: register budget for 6 or 8 waves per
 	                          SIMD (0 = the compiler's own allocation)        */
 	int fp32_queue;        /* FP32 walk: 1 = persistent work-queue kernel
-	                          (fixed grid, groups taken from a counter)       */
+	                          (fixed grid, groups taken from a counter); 0 =
+	                          one wave per group, or per work item for
+	                          split and captured walks                       */
 	int fp32_split;        /* FP32 walk with few active groups: parts each
 	                          group's subtree walks are cut into, one wave per
 	                          part (summed in part order); 0 = up to 64 parts,
