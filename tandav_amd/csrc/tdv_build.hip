@@ -370,6 +370,31 @@ __global__ void __launch_bounds__(256) k_tree_nodes(size_t n,
  * particles in PH order from LDS, every lane keeping the same sums, and the
  * cells that end at a particle (its level <= the deepest open level, found
  * by a ballot) take the sums at that particle. */
+/* min of v and the value DPP control `ctrl` moves into this lane; lanes
+ * that receive nothing (outside the row, or rows off in `rows`) see 99,
+ * above every level */
+template <int ctrl, int rows>
+__device__ __forceinline__ int dpp_min(int v)
+{
+	const int o = __builtin_amdgcn_update_dpp(99, v, ctrl, rows, 0xf, false);
+	return o < v ? o : v;
+}
+
+/* inclusive prefix minimum over the wavefront's 64 lanes in registers:
+ * row_shr 1, 2, 4, 8 within each row of 16, then row_bcast 15 (rows 1 and
+ * 3 take the last lane of the row before) and row_bcast 31 (rows 2 and 3
+ * take lane 31) */
+__device__ __forceinline__ int wave_prefix_min(int v)
+{
+	v = dpp_min<0x111, 0xf>(v);
+	v = dpp_min<0x112, 0xf>(v);
+	v = dpp_min<0x114, 0xf>(v);
+	v = dpp_min<0x118, 0xf>(v);
+	v = dpp_min<0x142, 0xa>(v);
+	v = dpp_min<0x143, 0xc>(v);
+	return v;
+}
+
 __global__ void __launch_bounds__(64) k_tree_nodes_long(
 		const uint64_t *__restrict__ key, const double *__restrict__ x,
 		const double *__restrict__ y, const double *__restrict__ z,
@@ -447,15 +472,10 @@ __global__ void __launch_bounds__(64) k_tree_nodes_long(
 				dn = k + 1 <= r.last ? div_level(ck0, ck1) : -1;
 			}
 			/* runmin before particle k: min of the levels of i .. k-1 */
-			int incl = dn;
-			for (int d = 1; d < 64; d <<= 1) {
-				const int o = __shfl_up(incl, d, 64);
-				if (lane >= d)
-					incl = o < incl ? o : incl;
-			}
-			int excl = __shfl_up(incl, 1, 64);
-			if (lane == 0)
-				excl = 99;
+			const int incl = wave_prefix_min(dn);
+			/* wave_shr 1: lane l takes lane l - 1's, lane 0 gets 99 */
+			const int excl = __builtin_amdgcn_update_dpp(99, incl, 0x138, 0xf,
+					0xf, false);
 			const int rm = runmin < excl ? runmin : excl;
 			/* not at e: the chain's last particle closes it before the
 			 * single-lane scan looks for a chain starting after it */
@@ -464,7 +484,7 @@ __global__ void __launch_bounds__(64) k_tree_nodes_long(
 				const int jp = base + (dn - 1 - lo);
 				up[jk] = jk - jp;
 			}
-			const int bmin = __shfl(incl, 63, 64);
+			const int bmin = __builtin_amdgcn_readlane(incl, 63);
 			runmin = runmin < bmin ? runmin : bmin;
 			sh[lane] = make_double4(tx, ty, tz, mk);
 			shdn[lane] = dn;
