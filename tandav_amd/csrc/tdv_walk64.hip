@@ -88,13 +88,28 @@ __device__ __forceinline__ void pp_range_on(Acc &a, const double4 *pm,
 }
 
 /* the smallest value over the wavefront, as a wave-uniform value */
+/* min of v and the value DPP control `ctrl` moves into this lane (INT_MAX
+ * where nothing arrives) */
+template <int ctrl, int rows>
+__device__ __forceinline__ int dpp_min(int v)
+{
+	const int o = __builtin_amdgcn_update_dpp(INT_MAX, v, ctrl, rows, 0xf,
+			false);
+	return o < v ? o : v;
+}
+
+/* The wavefront's minimum, in registers: a prefix minimum by DPP (row_shr
+ * 1, 2, 4, 8 within rows of 16, row_bcast 15 and 31 across rows) leaves
+ * it in lane 63.  Every lane of the wavefront takes part. */
 __device__ __forceinline__ int wave_min(int v)
 {
-	for (int d = 32; d >= 1; d >>= 1) {
-		const int o = __shfl_xor(v, d, 64);
-		v = o < v ? o : v;
-	}
-	return __builtin_amdgcn_readfirstlane(v);
+	v = dpp_min<0x111, 0xf>(v);
+	v = dpp_min<0x112, 0xf>(v);
+	v = dpp_min<0x114, 0xf>(v);
+	v = dpp_min<0x118, 0xf>(v);
+	v = dpp_min<0x142, 0xa>(v);
+	v = dpp_min<0x143, 0xc>(v);
+	return __builtin_amdgcn_readlane(v, 63);
 }
 
 /* One lane per active particle, every lane taking exactly the reference's
