@@ -445,6 +445,57 @@ def bench_domain(args, f32):
     print(json.dumps(line), flush=True)
 
 
+def bench_snapshot(args):
+    """Write_Snapshot and Read_Snapshot (IO/write_snapshot.c,
+    IO/read_snapshot.c) of the 128^3 box through the C ABI, in a local
+    temporary directory: particles/s each way and the file's MB/s, and the
+    read-back columns checked bit for bit against the written ones."""
+    import tempfile
+    from tandav_amd import Engine
+    pos, vel, m, soft, w = make_workload("c2")
+    n = len(m)
+    e = Engine(periodic=True, boxsize=1.0, softening=soft)
+    e.set_particles(pos, m, vel=vel, ids=np.arange(n))
+    rows = []
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "snap_000")
+        for rep in range(2):
+            t0 = time.perf_counter()
+            e.write_snapshot(path, npart=[0, n, 0, 0, 0, 0], boxsize=1.0)
+            t_w = time.perf_counter() - t0
+            size = os.path.getsize(path)
+            e2 = Engine(periodic=True, boxsize=1.0, softening=soft)
+            t0 = time.perf_counter()
+            e2.read_snapshot(path)
+            t_r = time.perf_counter() - t0
+            a, b = e.get("pos", "vel", "mass", "id"), e2.get("pos", "vel",
+                                                           "mass", "id")
+            same = all(np.array_equal(a[k], b[k]) for k in a)
+            e2.close()
+            rows.append({"write_s": t_w, "read_s": t_r, "bytes": size,
+                         "same": same})
+            log("snapshot: %d particles, %.1f MB: write %.3f s, read %.3f s,"
+                " same %s" % (n, size / 1e6, t_w, t_r, same))
+    best = min(rows, key=lambda r: r["write_s"] + r["read_s"])
+    line = {
+        "metric": "particles/s through Write_Snapshot + Read_Snapshot "
+                  "(Gadget format 2, one file)",
+        "value": n / (best["write_s"] + best["read_s"]),
+        "unit": "particles/s", "n_gpus": 1, "steps": len(rows), "warmup": 0,
+        "ms_per_step": (best["write_s"] + best["read_s"]) * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": w["desc"], "n_particles": n,
+                   "blocks": "HEAD POS VEL ID MASS ACC GACC",
+                   "parallelism": "single GPU, host file I/O"},
+        "write_MB_s": best["bytes"] / best["write_s"] / 1e6,
+        "read_MB_s": best["bytes"] / best["read_s"] / 1e6,
+        "bytes": best["bytes"], "runs": rows,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_c5(args, f32):
     """One full step of the large box with every particle active, device
     resident: PH keys, radix sort, reorder, top nodes, build, the FP32 walk
@@ -565,7 +616,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2",
-                    choices=sorted(WORKLOADS) + ["c4", "c5", "domain"])
+                    choices=sorted(WORKLOADS) + ["c4", "c5", "domain",
+                                                 "snapshot"])
     ap.add_argument("--ntargets", default="2,8,16,64",
                     help="target counts of the domain workload")
     ap.add_argument("--c4-n", type=int, default=0,
@@ -633,6 +685,9 @@ def main():
         return
     if args.workload == "domain":
         bench_domain(args, f32)
+        return
+    if args.workload == "snapshot":
+        bench_snapshot(args)
         return
 
     log("generating %s" % args.workload)
