@@ -185,12 +185,22 @@ int tdv_launch_walk(tdv_context *ctx);
  * order; acc_out = 3*nt doubles. */
 int tdv_forcetest(tdv_context *ctx, int nt, const int *idx, double *acc_out);
 
-/* tdv_forcetest with the Ewald correction of every pair summed exactly
- * (the Hernquist, Bouchet & Suto sums of Gravity/periodic.c:352-403
- * evaluated at the pair's separation) instead of read from the 65^3 table:
- * the periodic force without the table's interpolation error, for stating
- * the walks' error.  The same as tdv_forcetest on non-periodic runs. */
+/* tdv_forcetest with the exact periodic force instead of the 65^3 table's
+ * interpolation: the Ewald lattice sum of the Hernquist, Bouchet & Suto
+ * form (Gravity/periodic.c:352-403) that the table samples, here split at
+ * screening 4 (per pair: the nearest-image Newton term and the real-space
+ * images within 1.625 box; over all particles: the structure factors of
+ * the wave vectors with h^2 <= 63), converged below FP64 rounding.  The
+ * periodic force without the table's interpolation error, for stating the
+ * walks' error.  The same as tdv_forcetest on non-periodic runs. */
 int tdv_forcetest_exact(tdv_context *ctx, int nt, const int *idx,
+		double *acc_out);
+
+/* The same exact force with each pair's correction taken from the
+ * reference's own sums (screening 2, |n|, |h| <= 4 per component,
+ * Gravity/periodic.c:352-403) evaluated at the pair's separation: about
+ * 100 times slower than tdv_forcetest_exact, which it pins. */
+int tdv_forcetest_exact_pairs(tdv_context *ctx, int nt, const int *idx,
 		double *acc_out);
 
 /* Element-wise batched forms of the pure functions (host arrays). */
@@ -309,15 +319,90 @@ int tdv_make_active_list(tdv_context *ctx, int max_active_bin,
  * Drops a captured walk.  No counterpart in the reference. */
 int tdv_set_fp32_options(tdv_context *ctx, int waves, int queue, int split);
 
-/* Kick_First/Second_Halfstep (kick.c:9-70) for the active particles,
- * v += kick * Acc, then Drift_To_Sync_Point (drift.c:12-64) for every
- * particle, x += drift * v, wrapped into the box on periodic runs
+/* The KDK step of a run on one global time step (every particle active and
+ * in the same bin; otherwise TDV_ERR_ARG): Kick_First/Second_Halfstep
+ * (kick.c:9-70) as v += kick * Acc, then Drift_To_Sync_Point
+ * (drift.c:12-64) as x += drift * v, wrapped into the box on periodic runs
  * (Periodic_Constrain_Particles_To_Box, periodic.c:30-53), in FP64 on the
  * device; a zero factor skips its half.  kick and drift are the caller's
- * Particle_Kick_Step / Particle_Drift_Step (comoving.c:30-50).  The keys and
- * tree stay: sort again (or move the tree's monopoles) before the next
- * walk.  Timed as stage 7. */
+ * Particle_Kick_Step / Particle_Drift_Step (comoving.c:30-50).  Hierarchical
+ * steps, where each particle kicks and drifts from its own place on the
+ * integer timeline, are tdv_kick_halfstep and tdv_drift_to_sync_point.
+ * The keys and tree stay: sort again (or move the tree's monopoles) before
+ * the next walk.  Timed as stage 7. */
 int tdv_kick_drift(tdv_context *ctx, double kick, double drift);
+
+/* The integer timeline of the KDK step: Setup_Time_Integration
+ * (timestep.c:130-167; intime_t is 32 bits, globals.h:33, so the run is
+ * 2^31 ticks of Time.Step_Min, in ln a on comoving runs) and, on comoving
+ * runs, Setup_Comoving's kick and drift factor tables
+ * (setup_kick_drift_factors, comoving.c:109-153: 512 knots from 0.95
+ * time_begin to 1.05 time_end, natural cubic splines) over the background
+ * of cosmology.c:65-74. */
+typedef struct {
+	int comoving;          /* COMOVING (Config)                               */
+	double time_begin;     /* Time.Begin (expansion factor when comoving)     */
+	double time_end;       /* Time.End                                        */
+	double hubble;         /* Cosmo.Hubble_Constant in code units (comoving)  */
+	double omega_0, omega_matter, omega_lambda, omega_rad;  /* Cosmo.Omega_*  */
+} tdv_timeline;
+
+int tdv_set_timeline(tdv_context *ctx, const tdv_timeline *tl);
+
+/* setup_kick_drift_factors (comoving.c:109-153) on the host, no context:
+ * the 512 knots a[] (a[0] = 0, the knot the reference leaves unset) and the
+ * integrals of 1 / (H a^2) (kick[]) and 1 / (H a^3) (drift[]) from
+ * 0.95 time_begin, for a comoving timeline. */
+int tdv_comoving_tables(const tdv_timeline *tl, double *a, double *kick,
+		double *drift);
+
+/* P.It_Drift_Pos and P.It_Kick_Pos (particles.h:20-21), n uint32 each in
+ * the current particle order (NULL skips one).  tdv_set_particles sets both
+ * to Int_Time.Beg = 0; sorts carry them with the other columns. */
+int tdv_set_integer_times(tdv_context *ctx, const uint32_t *it_drift,
+		const uint32_t *it_kick);
+int tdv_get_integer_times(tdv_context *ctx, uint32_t *it_drift,
+		uint32_t *it_kick);
+
+/* Kick_First_Halfstep / Kick_Second_Halfstep (kick.c:9-70) for the active
+ * particles: each kicks by half of its own bin from its own It_Kick_Pos,
+ * v += Particle_Kick_Step(It_Kick_Pos, It_Kick_Pos + 2^Time_Bin / 2) * Acc,
+ * and advances It_Kick_Pos; FP64, the factor per particle on the device
+ * (comoving: the kick spline at both ends).  Timed as stage 7. */
+int tdv_kick_halfstep(tdv_context *ctx);
+
+/* Drift_To_Sync_Point (drift.c:12-64) for the active particles: each drifts
+ * from its own It_Drift_Pos to it_next (Int_Time.Next),
+ * x += Particle_Drift_Step(It_Drift_Pos, it_next) * v, It_Drift_Pos =
+ * it_next, wrapped into the box on periodic runs
+ * (Periodic_Constrain_Particles_To_Box, periodic.c:30-53; particles that
+ * do not move stay where they are).  The walk's packed positions follow.
+ * Timed as stage 7. */
+int tdv_drift_to_sync_point(tdv_context *ctx, uint32_t it_next);
+
+/* Inputs of set_new_particle_timebins (timestep.c:171-201). */
+typedef struct {
+	double time_int_accuracy;  /* Param.Time_Int_Accuracy                    */
+	double softening;          /* Param.Grav_Softening[1]                     */
+	double dt_max_global;      /* Dt_Max_Global (held as a float there)       */
+	int max_active_bin;        /* Time.Max_Active_Bin                         */
+	double expansion_factor;   /* Cosmo.Expansion_Factor (comoving)           */
+	double hubble_parameter;   /* Cosmo.Hubble_Parameter (comoving)           */
+	double grav_accel_factor;  /* Cosmo.Grav_Accel_Factor (comoving)          */
+} tdv_timestep_params;
+
+/* set_new_particle_timebins (timestep.c:171-201) on the device: every
+ * particle's P.Time_Bin from its step (cosmological_timestep on |Grav_Acc|,
+ * timestep.c:495-526, in the reference's float and double types, capped at
+ * dt_max_global, timestep2timebin :380-388), no longer than
+ * max(max_active_bin, its bin); *bin_min / *bin_max (may be NULL) get
+ * Time_Bin_Min / Time_Bin_Max.  A step below Time.Step_Min, or not finite,
+ * is TDV_ERR_RANGE (the reference's Assert, timestep.c:183-187).  The rest
+ * of Set_New_Timesteps is scalar (set_system_timestep, max_active_time_bin)
+ * and stays with the caller; tdv_make_active_list then gives the active
+ * list. */
+int tdv_set_new_timebins(tdv_context *ctx, const tdv_timestep_params *p,
+		int *bin_min, int *bin_max);
 
 /* Number of particles currently on the device. */
 size_t tdv_num_particles(const tdv_context *ctx);

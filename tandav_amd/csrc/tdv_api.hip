@@ -129,6 +129,8 @@ static void free_particles(tdv_context *c)
 	dfree(c->tree_parent);
 	dfree(c->time_bin);
 	dfree(c->id);
+	dfree(c->it_drift);
+	dfree(c->it_kick);
 	dfree(c->pm);
 	dfree(c->active);
 	for (int b = 0; b < 2; b++) {
@@ -166,6 +168,8 @@ static int alloc_particles(tdv_context *c, size_t n)
 	TDV_TRY(dalloc(c, &c->tree_parent, n));
 	TDV_TRY(dalloc(c, &c->time_bin, n));
 	TDV_TRY(dalloc(c, &c->id, n));
+	TDV_TRY(dalloc(c, &c->it_drift, n));
+	TDV_TRY(dalloc(c, &c->it_kick, n));
 	TDV_TRY(dalloc(c, &c->pm, n));
 	TDV_TRY(dalloc(c, &c->active, n));
 	for (int b = 0; b < 2; b++) {
@@ -345,6 +349,7 @@ void tdv_destroy(tdv_context *c)
 	dfree(c->node_c);
 	dfree(c->node_up);
 	dfree(c->ewald);
+	dfree(c->tl_spline);
 	dfree(c->ewald16);
 	dfree(c->ewald_s);
 	dfree(c->ewald_j);
@@ -418,6 +423,11 @@ int tdv_set_particles(tdv_context *c, size_t n, const double *pos[3],
 		TDV_HIP(c, hipMemsetAsync(c->last_acc, 0, b, c->stream));
 	TDV_HIP(c, hipMemsetAsync(c->cost, 0, b, c->stream));
 	TDV_HIP(c, hipMemsetAsync(c->tree_parent, 0, n * sizeof(int), c->stream));
+	/* It_Drift_Pos = It_Kick_Pos = Int_Time.Beg (timestep.c:137-153) */
+	TDV_HIP(c, hipMemsetAsync(c->it_drift, 0, n * sizeof(uint32_t),
+			c->stream));
+	TDV_HIP(c, hipMemsetAsync(c->it_kick, 0, n * sizeof(uint32_t),
+			c->stream));
 	if (time_bin)
 		TDV_HIP(c, hipMemcpyAsync(c->time_bin, time_bin, n * sizeof(int),
 				hipMemcpyHostToDevice, c->stream));
@@ -545,7 +555,8 @@ static int keys_sort_reorder(tdv_context *c)
 		c->tmp_f64 = t;
 	}
 	int *icols[] = { c->tree_parent, c->time_bin,
-		reinterpret_cast<int *>(c->id) };
+		reinterpret_cast<int *>(c->id), reinterpret_cast<int *>(c->it_drift),
+		reinterpret_cast<int *>(c->it_kick) };
 	for (int *col : icols) {
 		tdv_launch_gather_i32(c, n, c->perm, col, c->lohi);
 		TDV_HIP(c, hipMemcpyAsync(col, c->lohi, n * sizeof(int),
@@ -810,22 +821,30 @@ int tdv_launch_walk(tdv_context *c)
 	return tdv_check_err_flags(c);
 }
 
+enum { FT_TABLE, FT_EXACT_SPLIT, FT_EXACT_PAIRS };
+
 static int forcetest(tdv_context *c, int nt, const int *idx, double *acc_out,
-		bool exact);
+		int mode);
 
 int tdv_forcetest(tdv_context *c, int nt, const int *idx, double *acc_out)
 {
-	return forcetest(c, nt, idx, acc_out, false);
+	return forcetest(c, nt, idx, acc_out, FT_TABLE);
 }
 
 int tdv_forcetest_exact(tdv_context *c, int nt, const int *idx,
 		double *acc_out)
 {
-	return forcetest(c, nt, idx, acc_out, true);
+	return forcetest(c, nt, idx, acc_out, FT_EXACT_SPLIT);
+}
+
+int tdv_forcetest_exact_pairs(tdv_context *c, int nt, const int *idx,
+		double *acc_out)
+{
+	return forcetest(c, nt, idx, acc_out, FT_EXACT_PAIRS);
 }
 
 static int forcetest(tdv_context *c, int nt, const int *idx, double *acc_out,
-		bool exact)
+		int mode)
 {
 	if (!c || nt < 0 || (nt && (!idx || !acc_out)) || !c->n)
 		return tdv_fail(c, TDV_ERR_ARG, "tdv_forcetest: bad arguments");
@@ -843,7 +862,8 @@ static int forcetest(tdv_context *c, int nt, const int *idx, double *acc_out,
 		r = tdv_check(c, hipMemcpyAsync(di, idx, nt * sizeof(int),
 				hipMemcpyHostToDevice, c->stream), "copy");
 	if (!r)
-		r = exact ? tdv_direct_sum_exact(c, nt, di, dout)
+		r = mode == FT_EXACT_SPLIT ? tdv_direct_sum_split(c, nt, di, dout)
+				: mode == FT_EXACT_PAIRS ? tdv_direct_sum_exact(c, nt, di, dout)
 				: tdv_direct_sum(c, nt, di, dout);
 	if (!r)
 		r = tdv_check(c, hipMemcpyAsync(acc_out, dout, 3 * nt * sizeof(double),

@@ -334,3 +334,300 @@ int tdv_direct_sum_exact(tdv_context *c, int nt, const int *idx_dev,
 	(void)hipFree(part);
 	return r ? r : rs;
 }
+
+/*
+ * The same exact periodic force, cheaper: the Ewald split at a larger
+ * screening parameter.  For a unit mass at separation x (box units) the
+ * nearest-image Newton term plus the correction is the lattice sum
+ *   P(x) = sum_n (x - n) v(|x - n|) / |x - n|^3 + sum_{h != 0} c_h h sin(2 pi h.x),
+ *   v(r) = erfc(a r) + 2 a r / sqrt(pi) exp(-a^2 r^2),
+ *   c_h = 2 / h^2 exp(-pi^2 h^2 / a^2),
+ * whatever the screening a (the table's sums use a = 2 with |n|, |h| <= 4
+ * per component; both are converged far below FP64 rounding).  With a = 4
+ * the real-space images beyond |x - n| = 6.5 / a and the wave vectors with
+ * h^2 >= 64 each contribute below 2e-18, so they are left out.  The
+ * real-space part stays per pair (about 17 images); the Fourier part of all
+ * sources together is
+ *   sum_j m_j sin(2 pi h.(x_j - x_i)) = S_h cos(2 pi h.x_i) - C_h sin(2 pi h.x_i)
+ * with the structure factors S_h = sum_j m_j sin(2 pi h.x_j),
+ * C_h = sum_j m_j cos(2 pi h.x_j) formed once over all particles (half of
+ * the h, since the terms of h and -h are equal).  The target itself adds
+ * nothing in exact arithmetic (its real-space images cancel in pairs, its
+ * Fourier term is sin 0), so it is skipped in the pair sum as in
+ * k_direct_exact.
+ */
+#define SPLIT_ALPHA 4.0
+#define SPLIT_RC (6.5 / SPLIT_ALPHA)
+#define SPLIT_H2MAX 63
+#define SPLIT_HT 5                  /* wave vectors per thread (<= 1280) */
+
+/* x g(a r) / r^3, g(z) = erf(z) - 2 z / sqrt(pi) exp(-z^2): the n = 0
+ * real-space term minus Newton's x / r^3, as a series below a r = 0.5 */
+__device__ __forceinline__ double split_g_r3(double r2, double a)
+{
+	const double r = sqrt(r2);
+	const double z = a * r;
+	if (z < 0.5) {
+		const double z2 = z * z;
+		double term = 1.0, sum = 0.0, zp = 1.0;
+		for (int n = 1; n <= 14; n++) {
+			term = (n == 1) ? 1.0 : term / n;
+			const double c = ((n & 1) ? 1.0 : -1.0) * 2.0 * n / (2.0 * n + 1.0);
+			sum += c * term * zp;
+			zp *= z2;
+		}
+		return 2.0 / sqrt(EW_PI) * a * a * a * sum;
+	}
+	return (erf(z) - 2.0 * z / sqrt(EW_PI) * exp(-z * z)) / (r2 * r);
+}
+
+__global__ void __launch_bounds__(256) k_direct_split(int nt,
+		const int *__restrict__ idx, const double4 *__restrict__ pm, int n,
+		int chunk, double box, double G, double eps2, double eps3,
+		double *__restrict__ part)
+{
+	__shared__ double red[3][256];
+	const int t = blockIdx.x;
+	if (t >= nt)
+		return;
+	const int i = idx[t];
+	const double4 p = pm[i];
+	const double half = box / 2.0, ib = 1.0 / box;
+	const double a = SPLIT_ALPHA, rc = SPLIT_RC, rc2 = rc * rc;
+	const long long j0 = (long long)blockIdx.y * chunk;
+	const long long j1 = j0 + chunk < n ? j0 + chunk : n;
+	double ax = 0, ay = 0, az = 0;
+	for (long long jj = j0 + threadIdx.x; jj < j1; jj += blockDim.x) {
+		const int j = (int)jj;
+		if (j == i)
+			continue;
+		const double4 q = pm[j];
+		double d[3] = { q.x - p.x, q.y - p.y, q.z - p.z };
+		for (int k = 0; k < 3; k++) {
+			if (d[k] > half)
+				d[k] -= box;
+			else if (d[k] < -half)
+				d[k] += box;
+		}
+		const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+		if (r2 == 0)
+			continue;
+		double fac = G * q.w;
+		if (r2 < eps2) {
+			const double u2 = r2 / eps2;
+			fac *= (175 - u2 * (294 - u2 * 135)) / (16 * eps3);
+		} else {
+			const double ri = 1 / sqrt(r2);
+			fac *= ri * ri * ri;
+		}
+		const double x[3] = { d[0] * ib, d[1] * ib, d[2] * ib };
+		const double xr2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+		const double g0 = split_g_r3(xr2, a);
+		double e[3] = { -x[0] * g0, -x[1] * g0, -x[2] * g0 };
+		const int n0lo = (int)ceil(x[0] - rc), n0hi = (int)floor(x[0] + rc);
+		const int n1lo = (int)ceil(x[1] - rc), n1hi = (int)floor(x[1] + rc);
+		const int n2lo = (int)ceil(x[2] - rc), n2hi = (int)floor(x[2] + rc);
+		for (int u = n0lo; u <= n0hi; u++)
+			for (int v = n1lo; v <= n1hi; v++)
+				for (int w = n2lo; w <= n2hi; w++) {
+					if (u == 0 && v == 0 && w == 0)
+						continue;
+					const double y[3] = { x[0] - u, x[1] - v, x[2] - w };
+					const double yr2 = y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
+					if (yr2 >= rc2)
+						continue;
+					const double yr = sqrt(yr2);
+					const double val = (erfc(a * yr) + 2 * a * yr / sqrt(EW_PI)
+							* exp(-(a * yr) * (a * yr))) / (yr2 * yr);
+					e[0] += y[0] * val;
+					e[1] += y[1] * val;
+					e[2] += y[2] * val;
+				}
+		const double ge = G * q.w * ib * ib;
+		ax += fac * d[0] + ge * e[0];
+		ay += fac * d[1] + ge * e[1];
+		az += fac * d[2] + ge * e[2];
+	}
+	red[0][threadIdx.x] = ax;
+	red[1][threadIdx.x] = ay;
+	red[2][threadIdx.x] = az;
+	__syncthreads();
+	for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+		if ((int)threadIdx.x < s)
+			for (int d = 0; d < 3; d++)
+				red[d][threadIdx.x] += red[d][threadIdx.x + s];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		for (int d = 0; d < 3; d++)
+			part[((size_t)t * gridDim.y + blockIdx.y) * 3 + d] = red[d][0];
+}
+
+/* structure factors per chunk of particles: thread k takes wave vectors
+ * k, k + 256, ... (hv[h] = {hx, hy, hz, 0}); part[chunk * nh + h] = (S, C) */
+__global__ void __launch_bounds__(256) k_sfac_chunk(
+		const double4 *__restrict__ pm, long long n, long long chunk,
+		const int4 *__restrict__ hv, int nh, double tpib,
+		double2 *__restrict__ part)
+{
+	__shared__ double4 tile[256];
+	const long long j0 = (long long)blockIdx.x * chunk;
+	const long long j1 = j0 + chunk < n ? j0 + chunk : n;
+	double s[SPLIT_HT], cs[SPLIT_HT], hx[SPLIT_HT], hy[SPLIT_HT], hz[SPLIT_HT];
+#pragma unroll
+	for (int k = 0; k < SPLIT_HT; k++) {
+		const int h = threadIdx.x + 256 * k;
+		const int4 v = h < nh ? hv[h] : make_int4(0, 0, 0, 0);
+		hx[k] = v.x * tpib;
+		hy[k] = v.y * tpib;
+		hz[k] = v.z * tpib;
+		s[k] = 0;
+		cs[k] = 0;
+	}
+	for (long long b = j0; b < j1; b += 256) {
+		__syncthreads();
+		const long long j = b + threadIdx.x;
+		tile[threadIdx.x] = j < j1 ? pm[j] : make_double4(0, 0, 0, 0);
+		__syncthreads();
+		const int cnt = (int)(j1 - b < 256 ? j1 - b : 256);
+		for (int q = 0; q < cnt; q++) {
+			const double4 P = tile[q];
+#pragma unroll
+			for (int k = 0; k < SPLIT_HT; k++) {
+				double sn, c;
+				sincos(hx[k] * P.x + hy[k] * P.y + hz[k] * P.z, &sn, &c);
+				s[k] += P.w * sn;
+				cs[k] += P.w * c;
+			}
+		}
+	}
+#pragma unroll
+	for (int k = 0; k < SPLIT_HT; k++) {
+		const int h = threadIdx.x + 256 * k;
+		if (h < nh)
+			part[(size_t)blockIdx.x * nh + h] = make_double2(s[k], cs[k]);
+	}
+}
+
+/* chunk partials summed in chunk order, per wave vector */
+__global__ void k_sfac_sum(int nchunk, int nh, const double2 *__restrict__ part,
+		double2 *__restrict__ sf)
+{
+	const int h = blockIdx.x * blockDim.x + threadIdx.x;
+	if (h >= nh)
+		return;
+	double s = 0, c = 0;
+	for (int k = 0; k < nchunk; k++) {
+		const double2 v = part[(size_t)k * nh + h];
+		s += v.x;
+		c += v.y;
+	}
+	sf[h] = make_double2(s, c);
+}
+
+/* per target: the real-space chunks in order, then the Fourier part
+ * 2 sum_h c_h h (S_h cos b - C_h sin b), b = 2 pi h.x_i, times G / box^2 */
+__global__ void k_split_finish(int nt, int nchunk, const int *__restrict__ idx,
+		const double4 *__restrict__ pm, const double *__restrict__ part,
+		const int4 *__restrict__ hv, const double2 *__restrict__ sf, int nh,
+		double box, double G, double *__restrict__ out)
+{
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= nt)
+		return;
+	double a[3] = { 0, 0, 0 };
+	for (int ch = 0; ch < nchunk; ch++)
+		for (int d = 0; d < 3; d++)
+			a[d] += part[((size_t)t * nchunk + ch) * 3 + d];
+	const double4 p = pm[idx[t]];
+	const double tpib = 2.0 * EW_PI / box;
+	const double al = SPLIT_ALPHA;
+	double f[3] = { 0, 0, 0 };
+	for (int h = 0; h < nh; h++) {
+		const int4 v = hv[h];
+		const int h2 = v.x * v.x + v.y * v.y + v.z * v.z;
+		const double ch = 2.0 / h2 * exp(-EW_PI * EW_PI * h2 / (al * al));
+		double sn, cs;
+		sincos(tpib * (v.x * p.x + v.y * p.y + v.z * p.z), &sn, &cs);
+		const double w = 2.0 * ch * (sf[h].x * cs - sf[h].y * sn);
+		f[0] += w * v.x;
+		f[1] += w * v.y;
+		f[2] += w * v.z;
+	}
+	const double g = G / (box * box);
+	for (int d = 0; d < 3; d++)
+		out[3 * t + d] = a[d] + g * f[d];
+}
+
+int tdv_direct_sum_split(tdv_context *c, int nt, const int *idx_dev,
+		double *out_dev)
+{
+	if (nt <= 0)
+		return 0;
+	if (!c->par.periodic)
+		return tdv_direct_sum(c, nt, idx_dev, out_dev);
+	/* wave vectors of the half space h > 0 (lexicographic), h^2 <= 63 */
+	std::vector<int4> hv;
+	for (int x = 0; x <= 7; x++)
+		for (int y = -7; y <= 7; y++)
+			for (int z = -7; z <= 7; z++) {
+				const int h2 = x * x + y * y + z * z;
+				if (h2 == 0 || h2 > SPLIT_H2MAX)
+					continue;
+				if (x == 0 && (y < 0 || (y == 0 && z < 0)))
+					continue;
+				hv.push_back(make_int4(x, y, z, 0));
+			}
+	const int nh = (int)hv.size();
+	if (nh > 256 * SPLIT_HT)
+		return tdv_fail(c, TDV_ERR_ARG, "too many wave vectors");
+	const size_t n = c->n;
+	size_t nchunk = (4096 + (size_t)nt - 1) / (size_t)nt;
+	if (nchunk > (n + 4095) / 4096)
+		nchunk = (n + 4095) / 4096;
+	if (nchunk < 1)
+		nchunk = 1;
+	if (nchunk > 65535)
+		nchunk = 65535;
+	const size_t chunk = (n + nchunk - 1) / nchunk;
+	/* structure-factor chunks: about 2048 workgroups */
+	const size_t schunk = ((n + 2047) / 2048 + 255) / 256 * 256;
+	const size_t nsch = (n + schunk - 1) / schunk;
+	double *part = nullptr;
+	int4 *dh = nullptr;
+	double2 *spart = nullptr, *sf = nullptr;
+	int r = tdv_check(c, hipMalloc((void **)&part,
+			(size_t)nt * nchunk * 3 * sizeof(double)), "hipMalloc");
+	if (!r)
+		r = tdv_check(c, hipMalloc((void **)&dh, nh * sizeof(int4)), "hipMalloc");
+	if (!r)
+		r = tdv_check(c, hipMalloc((void **)&spart,
+				nsch * nh * sizeof(double2)), "hipMalloc");
+	if (!r)
+		r = tdv_check(c, hipMalloc((void **)&sf, nh * sizeof(double2)),
+				"hipMalloc");
+	if (!r)
+		r = tdv_check(c, hipMemcpyAsync(dh, hv.data(), nh * sizeof(int4),
+				hipMemcpyHostToDevice, c->stream), "copy");
+	if (!r) {
+		hipLaunchKernelGGL(k_sfac_chunk, dim3((unsigned)nsch), dim3(256), 0,
+				c->stream, c->pm, (long long)n, (long long)schunk, dh, nh,
+				2.0 * EW_PI / c->par.boxsize, spart);
+		hipLaunchKernelGGL(k_sfac_sum, dim3(tdv_blocks(nh, 256)), dim3(256), 0,
+				c->stream, (int)nsch, nh, spart, sf);
+		hipLaunchKernelGGL(k_direct_split, dim3(nt, (unsigned)nchunk),
+				dim3(256), 0, c->stream, nt, idx_dev, c->pm, (int)n, (int)chunk,
+				c->par.boxsize, c->par.gravity, c->Epsilon2, c->Epsilon3, part);
+		hipLaunchKernelGGL(k_split_finish, dim3(tdv_blocks((size_t)nt, 64)),
+				dim3(64), 0, c->stream, nt, (int)nchunk, idx_dev, c->pm, part,
+				dh, sf, nh, c->par.boxsize, c->par.gravity, out_dev);
+		r = tdv_check(c, hipGetLastError(), "k_direct_split");
+	}
+	const int rs = tdv_check(c, hipStreamSynchronize(c->stream),
+			"k_direct_split");
+	(void)hipFree(part);
+	(void)hipFree(dh);
+	(void)hipFree(spart);
+	(void)hipFree(sf);
+	return r ? r : rs;
+}

@@ -68,6 +68,7 @@ struct tdv_context {
 	uint64_t *key_tmp;
 	int *tree_parent, *time_bin;
 	uint32_t *id;
+	uint32_t *it_drift, *it_kick;    /* P.It_Drift_Pos, P.It_Kick_Pos */
 	double4 *pm;                     /* {x, y, z, m} for the walk */
 	int *active;                     /* active list */
 	int nactive;
@@ -137,6 +138,12 @@ struct tdv_context {
 
 	hipEvent_t ev[16];
 	float stage_ms[8];
+
+	/* integer timeline of the KDK step (tdv_integrate.hip) */
+	tdv_timeline tl;
+	double tl_step_min, tl_step_max;
+	double *tl_spline;               /* 5 x 512: knots, kick y, c, drift y, c */
+	int have_timeline;
 };
 
 /* ---- launchers (tdv_keys_sort.hip) ---- */
@@ -180,6 +187,8 @@ void tdv_graph_drop(tdv_context *c);
 #define TDV_SPLIT_AUTO_MAX 64            /* parts per group, split 0 */
 int tdv_direct_sum(tdv_context *c, int nt, const int *idx_dev, double *out_dev);
 int tdv_direct_sum_exact(tdv_context *c, int nt, const int *idx_dev,
+		double *out_dev);
+int tdv_direct_sum_split(tdv_context *c, int nt, const int *idx_dev,
 		double *out_dev);
 void tdv_launch_safe_last_accel(tdv_context *c);
 int tdv_ewald_table_device(tdv_context *c, double *out4_unnormalised);

@@ -212,20 +212,28 @@ __global__ void __launch_bounds__(SC_THREADS) k_scan_down(
 	}
 }
 
-int tdv_scan_u32(tdv_context *c, const uint32_t *in, uint32_t *out, size_t n)
+/* exclusive scan on stream s with the block partials in part
+ * ((n + SC_TILE - 1) / SC_TILE + 1 entries); errors go to c (may be NULL) */
+static int scan_u32(tdv_context *c, hipStream_t s, uint32_t *part,
+		const uint32_t *in, uint32_t *out, size_t n)
 {
 	const size_t nb = (n + SC_TILE - 1) / SC_TILE;
 	if (nb == 0) {
-		TDV_HIP(c, hipMemsetAsync(out, 0, sizeof(uint32_t), c->stream));
+		TDV_HIP(c, hipMemsetAsync(out, 0, sizeof(uint32_t), s));
 		return 0;
 	}
 	hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SC_THREADS), 0,
-			c->stream, in, n, c->scan_part);
-	hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(SC_THREADS), 0,
-			c->stream, c->scan_part, nb, out + n);
+			s, in, n, part);
+	hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(SC_THREADS), 0, s, part,
+			nb, out + n);
 	hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(SC_THREADS), 0,
-			c->stream, in, n, c->scan_part, out);
+			s, in, n, part, out);
 	return tdv_check(c, hipGetLastError(), "scan");
+}
+
+int tdv_scan_u32(tdv_context *c, const uint32_t *in, uint32_t *out, size_t n)
+{
+	return scan_u32(c, c->stream, c->scan_part, in, out, n);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -324,29 +332,61 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(
 	}
 }
 
+/* the buffers a key sort works in */
+struct SortWork {
+	hipStream_t stream;
+	uint64_t *sk[2];             /* n keys each */
+	uint32_t *sv[2];             /* n payloads each */
+	uint32_t *hist;              /* hist_cap digit counts */
+	size_t hist_cap;
+	uint32_t *scan_part;         /* scan block partials */
+	unsigned *d_err;             /* TDV_EBIT_* flags */
+	uint64_t *hi_tmp;            /* n words: the tie fallback's high words */
+	uint32_t *p_tmp;             /* n: the tie fallback's first order */
+};
+
+static SortWork context_work(tdv_context *c, const uint32_t *perm_out)
+{
+	SortWork w;
+	w.stream = c->stream;
+	for (int b = 0; b < 2; b++) {
+		w.sk[b] = c->sk[b];
+		w.sv[b] = c->sv[b];
+	}
+	w.hist = c->hist;
+	w.hist_cap = c->hist_cap;
+	w.scan_part = c->scan_part;
+	w.d_err = c->d_err;
+	w.hi_tmp = reinterpret_cast<uint64_t *>(c->tmp_f64);
+	w.p_tmp = c->perm == perm_out ? reinterpret_cast<uint32_t *>(c->lohi)
+			: c->perm;
+	return w;
+}
+
 /* sorts n u64 keys (read with stride/offset, e.g. the high words of 128-bit
- * keys) carrying payload vin (NULL = original index) into c->sk/sv; returns
+ * keys) carrying payload vin (NULL = original index) into w.sk/sv; returns
  * the buffer index holding the result */
-static int radix_sort_u64(tdv_context *c, size_t n, const uint64_t *keys,
-		int stride, int off, const uint32_t *vin, int *res)
+static int radix_sort_u64(tdv_context *c, const SortWork &w, size_t n,
+		const uint64_t *keys, int stride, int off, const uint32_t *vin,
+		int *res)
 {
 	const unsigned nb = (unsigned)((n + RS_TILE - 1) / RS_TILE);
 	const size_t hn = (size_t)nb * RS_BINS;
-	if (hn + 1 > c->hist_cap)
+	if (hn + 1 > w.hist_cap)
 		return tdv_fail(c, TDV_ERR_ARG, "radix sort histogram too small");
 	const uint64_t *kin = keys;
 	const uint32_t *vi = vin;
 	int s = stride, o = off, cur = 0;
 	for (int pass = 0; pass < 8; pass++) {
 		const int shift = 8 * pass;
-		hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_THREADS), 0, c->stream,
-				kin, s, o, n, shift, c->hist, nb);
-		TDV_TRY(tdv_scan_u32(c, c->hist, c->hist, hn));
+		hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_THREADS), 0, w.stream,
+				kin, s, o, n, shift, w.hist, nb);
+		TDV_TRY(scan_u32(c, w.stream, w.scan_part, w.hist, w.hist, hn));
 		hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_THREADS), 0,
-				c->stream, kin, s, o, vi, c->sk[cur], c->sv[cur], n, shift,
-				c->hist, nb);
-		kin = c->sk[cur];
-		vi = c->sv[cur];
+				w.stream, kin, s, o, vi, w.sk[cur], w.sv[cur], n, shift,
+				w.hist, nb);
+		kin = w.sk[cur];
+		vi = w.sv[cur];
 		s = 1;
 		o = 0;
 		cur ^= 1;
@@ -410,39 +450,47 @@ __global__ void k_check_dups(size_t n, const uint32_t *__restrict__ perm,
 		atomicOr(err, TDV_EBIT_DUP);
 }
 
-int tdv_sort_device(tdv_context *c, size_t n, const uint64_t *keys,
-		uint32_t *perm_out)
+/* Qsort_Index's order of unique 128-bit keys (2n words) into perm_out:
+ * the high words radix-sorted, runs of equal high words ordered by the low
+ * word; runs longer than 4096 switch to a full 128-bit LSD sort.  Duplicate
+ * keys set TDV_EBIT_DUP in w.d_err.  Reads w.d_err back (a stream
+ * synchronisation) to choose the fallback. */
+static int sort_keys_work(tdv_context *c, const SortWork &w, size_t n,
+		const uint64_t *keys, uint32_t *perm_out)
 {
 	if (n == 0)
 		return 0;
 	int res;
-	TDV_TRY(radix_sort_u64(c, n, keys, 2, 1, nullptr, &res));
+	TDV_TRY(radix_sort_u64(c, w, n, keys, 2, 1, nullptr, &res));
 	hipLaunchKernelGGL(k_fix_ties, dim3(tdv_blocks(n, 256)), dim3(256), 0,
-			c->stream, n, c->sk[res], c->sv[res], keys, c->d_err);
+			w.stream, n, w.sk[res], w.sv[res], keys, w.d_err);
 	unsigned e = 0;
-	TDV_HIP(c, hipMemcpyAsync(&e, c->d_err, sizeof(e), hipMemcpyDeviceToHost,
-			c->stream));
-	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	TDV_HIP(c, hipMemcpyAsync(&e, w.d_err, sizeof(e), hipMemcpyDeviceToHost,
+			w.stream));
+	TDV_HIP(c, hipStreamSynchronize(w.stream));
 	if (e & TDV_EBIT_TIES) {
 		/* long runs of equal high words: full 128-bit LSD sort, low word
 		 * first (8 passes), then the high words in that order (8 passes) */
-		TDV_HIP(c, hipMemsetAsync(c->d_err, 0, sizeof(unsigned), c->stream));
+		TDV_HIP(c, hipMemsetAsync(w.d_err, 0, sizeof(unsigned), w.stream));
 		int r1;
-		TDV_TRY(radix_sort_u64(c, n, keys, 2, 0, nullptr, &r1));
-		uint64_t *hi = reinterpret_cast<uint64_t *>(c->tmp_f64);
-		uint32_t *p1 = c->perm == perm_out ? reinterpret_cast<uint32_t *>(c->lohi)
-				: c->perm;
-		TDV_HIP(c, hipMemcpyAsync(p1, c->sv[r1], n * sizeof(uint32_t),
-				hipMemcpyDeviceToDevice, c->stream));
+		TDV_TRY(radix_sort_u64(c, w, n, keys, 2, 0, nullptr, &r1));
+		TDV_HIP(c, hipMemcpyAsync(w.p_tmp, w.sv[r1], n * sizeof(uint32_t),
+				hipMemcpyDeviceToDevice, w.stream));
 		hipLaunchKernelGGL(k_gather_hi, dim3(tdv_blocks(n, 256)), dim3(256), 0,
-				c->stream, n, p1, keys, hi);
-		TDV_TRY(radix_sort_u64(c, n, hi, 1, 0, p1, &res));
+				w.stream, n, w.p_tmp, keys, w.hi_tmp);
+		TDV_TRY(radix_sort_u64(c, w, n, w.hi_tmp, 1, 0, w.p_tmp, &res));
 		hipLaunchKernelGGL(k_check_dups, dim3(tdv_blocks(n, 256)), dim3(256), 0,
-				c->stream, n, c->sv[res], keys, c->d_err);
+				w.stream, n, w.sv[res], keys, w.d_err);
 	}
-	TDV_HIP(c, hipMemcpyAsync(perm_out, c->sv[res], n * sizeof(uint32_t),
-			hipMemcpyDeviceToDevice, c->stream));
+	TDV_HIP(c, hipMemcpyAsync(perm_out, w.sv[res], n * sizeof(uint32_t),
+			hipMemcpyDeviceToDevice, w.stream));
 	return tdv_check(c, hipGetLastError(), "sort");
+}
+
+int tdv_sort_device(tdv_context *c, size_t n, const uint64_t *keys,
+		uint32_t *perm_out)
+{
+	return sort_keys_work(c, context_work(c, perm_out), n, keys, perm_out);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -544,7 +592,8 @@ int tdv_domain_nonperiodic(tdv_context *c)
 		hipLaunchKernelGGL(k_f64_orderkey, dim3(tdv_blocks(m, 256)), dim3(256),
 				0, c->stream, m, c->pos[d] + 1, tmp);
 		int res;
-		TDV_TRY(radix_sort_u64(c, m, tmp, 1, 0, nullptr, &res));
+		TDV_TRY(radix_sort_u64(c, context_work(c, nullptr), m, tmp, 1, 0,
+				nullptr, &res));
 		uint64_t u;
 		TDV_HIP(c, hipMemcpyAsync(&u, c->sk[res] + k, 8, hipMemcpyDeviceToHost,
 				c->stream));

@@ -1,13 +1,14 @@
 /*
- * Kick and drift of the KDK step on the device, in FP64.
- *
- * Kick_First_Halfstep / Kick_Second_Halfstep (kick.c:9-70): the active
- * particles' velocities take v += dt_kick a; Drift_To_Sync_Point
- * (drift.c:12-64): every particle's position takes x += dt_drift v, then
- * Periodic_Constrain_Particles_To_Box (periodic.c:30-53) wraps it into the
- * box.  The factors are the caller's: Particle_Kick_Step and
- * Particle_Drift_Step (comoving.c:30-50 in comoving runs, the plain time
- * difference otherwise).  The walk's packed positions follow the drift.
+ * The KDK step of a run on one global time step, on the device, in FP64:
+ * every particle is active and in the same time bin, so the half-step kicks
+ * (kick.c:9-70) and the drift (drift.c:12-64) take one factor each, the
+ * caller's Particle_Kick_Step / Particle_Drift_Step (comoving.c:30-50 on
+ * comoving runs, the plain time difference otherwise): v += dt_kick a,
+ * x += dt_drift v, then Periodic_Constrain_Particles_To_Box
+ * (periodic.c:30-53).  The walk's packed positions follow the drift.
+ * Hierarchical steps, where each particle kicks and drifts from its own
+ * place on the integer timeline, are tdv_kick_halfstep and
+ * tdv_drift_to_sync_point (tdv_integrate.hip).
  */
 #include "tdv_internal.h"
 
@@ -66,6 +67,10 @@ extern "C" int tdv_kick_drift(tdv_context *c, double kick, double drift)
 	if (!c || !c->n || !std::isfinite(kick) || !std::isfinite(drift))
 		return tdv_fail(c, TDV_ERR_ARG, "tdv_kick_drift: no particles or "
 				"factors not finite");
+	if ((size_t)c->nactive != c->n)
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_kick_drift: one global step "
+				"needs every particle active; hierarchical steps use "
+				"tdv_kick_halfstep and tdv_drift_to_sync_point");
 	(void)hipEventRecord(c->ev[EV_KICK0], c->stream);
 	if (kick != 0 && c->nactive)
 		hipLaunchKernelGGL(k_kick, dim3(tdv_blocks((size_t)c->nactive, 256)),

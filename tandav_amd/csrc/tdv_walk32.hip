@@ -66,10 +66,14 @@ __device__ __forceinline__ bool rel_open(float m, float s, float r2,
 
 /* ---- preparation: fixed-point particle / node records -------------------- */
 
+/* fixed-point coordinate modulo 2^32: floor((x - o) S) through a signed
+ * 64-bit integer (a position that drifted below the origin gives a
+ * negative value, which the conversion to unsigned wraps, as the walk's
+ * modular differences expect) */
 __device__ __forceinline__ unsigned fixp(double x, double o, double S)
 {
-	const double u = (x - o) * S;
-	return (unsigned)(unsigned long long)u;
+	const double u = floor((x - o) * S);
+	return (unsigned)(long long)u;
 }
 
 

@@ -44,6 +44,48 @@ class TdvParams(ctypes.Structure):
                 ("fp32_split", ctypes.c_int)]
 
 
+class TdvTimeline(ctypes.Structure):
+    """tdv_timeline: Time.Begin/End and the comoving background."""
+    _fields_ = [("comoving", ctypes.c_int), ("time_begin", ctypes.c_double),
+                ("time_end", ctypes.c_double), ("hubble", ctypes.c_double),
+                ("omega_0", ctypes.c_double),
+                ("omega_matter", ctypes.c_double),
+                ("omega_lambda", ctypes.c_double),
+                ("omega_rad", ctypes.c_double)]
+
+
+class TdvTimestepParams(ctypes.Structure):
+    """tdv_timestep_params: inputs of set_new_particle_timebins."""
+    _fields_ = [("time_int_accuracy", ctypes.c_double),
+                ("softening", ctypes.c_double),
+                ("dt_max_global", ctypes.c_double),
+                ("max_active_bin", ctypes.c_int),
+                ("expansion_factor", ctypes.c_double),
+                ("hubble_parameter", ctypes.c_double),
+                ("grav_accel_factor", ctypes.c_double)]
+
+
+def timeline(comoving=False, time_begin=0.0, time_end=1.0, hubble=1.0,
+             omega_matter=0.3, omega_lambda=0.7, omega_rad=0.0,
+             omega_0=None):
+    if omega_0 is None:
+        omega_0 = omega_matter + omega_lambda + omega_rad
+    return TdvTimeline(int(comoving), float(time_begin), float(time_end),
+                       float(hubble), float(omega_0), float(omega_matter),
+                       float(omega_lambda), float(omega_rad))
+
+
+def comoving_tables(tl):
+    """setup_kick_drift_factors (comoving.c:109-153) on the host: the 512
+    knots and the kick and drift integrals (no GPU needed)."""
+    L = load_library()
+    a, k, d = (np.zeros(512) for _ in range(3))
+    rc = L.tdv_comoving_tables(ctypes.byref(tl), _d(a), _d(k), _d(d))
+    if rc:
+        raise TDVError("tdv_comoving_tables: error %d" % rc)
+    return a, k, d
+
+
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
 _sz = ctypes.c_size_t
@@ -88,6 +130,7 @@ def load_library(path=LIB_PATH):
     L.tdv_compute_acceleration.argtypes = [_ctx, ctypes.c_int]
     L.tdv_forcetest.argtypes = [_ctx, ctypes.c_int, _vp, _dp]
     L.tdv_forcetest_exact.argtypes = [_ctx, ctypes.c_int, _vp, _dp]
+    L.tdv_forcetest_exact_pairs.argtypes = [_ctx, ctypes.c_int, _vp, _dp]
     L.tdv_peano_keys.argtypes = [_ctx, _sz, _dp, _dp, _dp, _dp,
                                  ctypes.c_double, ctypes.c_int, _vp]
     L.tdv_sort_keys.argtypes = [_ctx, _sz, _vp, _vp]
@@ -117,6 +160,17 @@ def load_library(path=LIB_PATH):
     L.tdv_set_fp32_options.argtypes = [_ctx, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int]
     L.tdv_capture_walk.argtypes = [_ctx]
+    L.tdv_set_timeline.argtypes = [_ctx, ctypes.POINTER(TdvTimeline)]
+    L.tdv_comoving_tables.argtypes = [ctypes.POINTER(TdvTimeline), _dp, _dp,
+                                      _dp]
+    L.tdv_set_integer_times.argtypes = [_ctx, _vp, _vp]
+    L.tdv_get_integer_times.argtypes = [_ctx, _vp, _vp]
+    L.tdv_kick_halfstep.argtypes = [_ctx]
+    L.tdv_drift_to_sync_point.argtypes = [_ctx, ctypes.c_uint32]
+    L.tdv_set_new_timebins.argtypes = [_ctx,
+                                       ctypes.POINTER(TdvTimestepParams),
+                                       ctypes.POINTER(ctypes.c_int),
+                                       ctypes.POINTER(ctypes.c_int)]
     L.tdv_launch_walk.argtypes = [_ctx]
     L.tdv_write_snapshot.argtypes = [_ctx, ctypes.c_char_p,
                                      ctypes.POINTER(SnapshotHeader),
@@ -380,13 +434,16 @@ class Engine:
         """Replay the captured walk."""
         self._ok(self.lib.tdv_launch_walk(self._c), "tdv_launch_walk")
 
-    def forcetest_exact(self, idx):
-        """forcetest with every pair's Ewald correction summed exactly
-        instead of interpolated from the table."""
+    def forcetest_exact(self, idx, pairs=False):
+        """forcetest with the exact periodic force (the Ewald lattice sum
+        the table samples) instead of the table's interpolation; pairs=True
+        evaluates the reference's sums per pair (slow, the pin of the
+        split form)."""
         idx = np.ascontiguousarray(idx, dtype=np.int32)
         out = np.zeros((idx.size, 3))
-        self._ok(self.lib.tdv_forcetest_exact(self._c, idx.size,
-                                              idx.ctypes.data_as(_vp), _d(out)),
+        f = (self.lib.tdv_forcetest_exact_pairs if pairs
+             else self.lib.tdv_forcetest_exact)
+        self._ok(f(self._c, idx.size, idx.ctypes.data_as(_vp), _d(out)),
                  "tdv_forcetest_exact")
         return out
 
@@ -573,10 +630,62 @@ class Engine:
         p.fp32_waves, p.fp32_queue, p.fp32_split = w, q, s
 
     def kick_drift(self, kick, drift):
-        """Kick the active particles (v += kick a), then drift every particle
-        (x += drift v, wrapped into the box when periodic), on the device."""
+        """The KDK step of a run on one global time step (every particle
+        active): v += kick a, then x += drift v (wrapped into the box when
+        periodic), on the device."""
         self._ok(self.lib.tdv_kick_drift(self._c, float(kick), float(drift)),
                  "tdv_kick_drift")
+
+    # -- integer timeline (kick.c, drift.c, timestep.c, comoving.c) ----------
+    def set_timeline(self, tl):
+        """Setup_Time_Integration + Setup_Comoving for a tdv_timeline."""
+        self._ok(self.lib.tdv_set_timeline(self._c, ctypes.byref(tl)),
+                 "tdv_set_timeline")
+
+    def set_integer_times(self, it_drift=None, it_kick=None):
+        """P.It_Drift_Pos / P.It_Kick_Pos in the current particle order."""
+        a = (None if it_drift is None
+             else np.ascontiguousarray(it_drift, dtype=np.uint32))
+        b = (None if it_kick is None
+             else np.ascontiguousarray(it_kick, dtype=np.uint32))
+        self._ok(self.lib.tdv_set_integer_times(
+            self._c, a.ctypes.data_as(_vp) if a is not None else None,
+            b.ctypes.data_as(_vp) if b is not None else None),
+            "tdv_set_integer_times")
+
+    def integer_times(self):
+        a = np.zeros(self.n, np.uint32)
+        b = np.zeros(self.n, np.uint32)
+        self._ok(self.lib.tdv_get_integer_times(
+            self._c, a.ctypes.data_as(_vp), b.ctypes.data_as(_vp)),
+            "tdv_get_integer_times")
+        return a, b
+
+    def kick_halfstep(self):
+        """Kick_First/Second_Halfstep (kick.c:9-70) on the device."""
+        self._ok(self.lib.tdv_kick_halfstep(self._c), "tdv_kick_halfstep")
+
+    def drift_to_sync_point(self, it_next):
+        """Drift_To_Sync_Point (drift.c:12-64) to Int_Time.Next = it_next."""
+        self._ok(self.lib.tdv_drift_to_sync_point(self._c, int(it_next)),
+                 "tdv_drift_to_sync_point")
+
+    def set_new_timebins(self, time_int_accuracy, softening, dt_max_global,
+                         max_active_bin, expansion_factor=1.0,
+                         hubble_parameter=1.0, grav_accel_factor=1.0):
+        """set_new_particle_timebins (timestep.c:171-201) on the device;
+        returns (Time_Bin_Min, Time_Bin_Max)."""
+        p = TdvTimestepParams(float(time_int_accuracy), float(softening),
+                              float(dt_max_global), int(max_active_bin),
+                              float(expansion_factor),
+                              float(hubble_parameter),
+                              float(grav_accel_factor))
+        lo, hi = ctypes.c_int(), ctypes.c_int()
+        self._ok(self.lib.tdv_set_new_timebins(self._c, ctypes.byref(p),
+                                               ctypes.byref(lo),
+                                               ctypes.byref(hi)),
+                 "tdv_set_new_timebins")
+        return lo.value, hi.value
 
     def walk32_quadrupoles(self):
         """The FP32 walk's quadrupole records of the last FP32 walk: (nodes

@@ -1,0 +1,171 @@
+"""The benchmark configurations at their stated sizes, through the C ABI.
+
+* 128^3 periodic Zel'dovich box, FP64 walk (the drop-in): PH keys and the
+  sort order bit-exact against the oracle on all 2,097,152 particles, and
+  on 256 sampled particles the oracle's regular-step walk (relative
+  criterion 0.005 on the GPU's |a| of the prepare step, Ewald walk
+  included) takes the same interactions (Cost identical) and gives the
+  same acceleration to 1e-6 relative (the drop-in bar).
+* 256^3 box, FP32 group walk at opening criteria 0.0025, 0.005 and 0.01:
+  p99 |da|/|a| <= 2e-3 against FP64 direct summation with the exact
+  periodic force on 512 sampled particles.
+* Clustered 2^23 box (256 Hernquist halos + 20% background), FP32 walk at
+  alpha 0.003: eager walks and replays of the captured walk graph at 100%,
+  3.8% and 0.05% of the particles active (time bins by |a| rank), the
+  replay bit-identical to the eager walk and p99 <= 2e-3 against the exact
+  force on 256 sampled active particles per set.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from tandav_amd import Engine, ics
+
+pytestmark = pytest.mark.gpu
+
+BAR = 2e-3
+
+
+def rel_err(a, ref):
+    return np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def _fp32(**kw):
+    base = dict(periodic=True, boxsize=1.0, theta_bh=0.1, compat=False,
+                precision="fp32", ewald_size=0.125, ewald_accuracy=1.0,
+                fp32_order=2, fp32_criterion=3)
+    base.update(kw)
+    return Engine(**base)
+
+
+def test_c2_fp64_walk_matches_oracle_at_full_size():
+    ng = 128
+    pos, _, m = ics.zeldovich_box(ng, seed=7)
+    soft = 1.0 / (40 * ng)
+    n = len(m)
+    table = oracle.cached_ewald_table()
+    say("c2: %d particles, GPU prepare step + regular step" % n)
+    e = Engine(periodic=True, softening=soft, theta_bh=0.1, alpha_rel=0.005,
+               compat=True, ewald_table=table)
+    e.set_particles(pos, m)
+    e.setup_domain()
+    e.domain_decomposition()
+    e.compute_acceleration(True)
+    e.domain_decomposition()
+    e.compute_acceleration(False)
+    got = e.get("acc", "cost", "last_acc_mag", "key", "pos")
+    e.close()
+    say("c2: oracle decomposition and build")
+    s = oracle.Sim(pos, m, periodic=True, softening=soft, theta_bh=0.1,
+                   alpha_rel=0.005, ewald_table=table,
+                   nthreads=min(16, os.cpu_count() or 1))
+    s.setup_domain()
+    s.domain_decomposition()
+    np.testing.assert_array_equal(got["key"], s.key)
+    np.testing.assert_array_equal(got["pos"], s.positions())
+    s.tree_build()
+    g = np.random.Generator(np.random.Philox(99))
+    pick = np.sort(g.choice(n, 256, replace=False))
+    s.time_bin[:] = 1
+    s.time_bin[pick] = 0
+    s.set_active_bin(0)
+    s.last_acc_mag[:] = got["last_acc_mag"]
+    say("c2: oracle walk on 256 particles")
+    s.tree_acceleration(use_bh=False)
+    np.testing.assert_array_equal(s.cost[pick], got["cost"][pick])
+    err = rel_err(got["acc"][pick], s.accelerations()[pick])
+    say("c2: max |da|/|a| = %.3g" % err.max())
+    assert err.max() <= 1e-6
+
+
+@pytest.fixture(scope="module")
+def box256():
+    pos, _, m = ics.zeldovich_box(256, seed=7)
+    return pos, m
+
+
+@pytest.mark.parametrize("alpha", [0.01, 0.005, 0.0025])
+def test_c3_fp32_walk_error_at_full_size(box256, alpha):
+    pos, m = box256
+    n = len(m)
+    soft = 1.0 / (40 * 256)
+    say("c3: alpha %g, %d particles" % (alpha, n))
+    e = _fp32(softening=soft, alpha_rel=alpha)
+    e.set_particles(pos, m)
+    e.setup_domain()
+    e.domain_decomposition()
+    e.compute_acceleration(True)
+    e.domain_decomposition()
+    e.compute_acceleration(False)
+    acc = e.get("acc")["acc"]
+    g = np.random.Generator(np.random.Philox(6))
+    idx = np.sort(g.choice(n, 512, replace=False))
+    say("c3: exact periodic force on 512 particles")
+    ref = e.forcetest_exact(idx)
+    e.close()
+    err = rel_err(acc[idx], ref)
+    say("c3: alpha %g median %.3g p99 %.3g" % (alpha, np.median(err),
+                                                np.percentile(err, 99)))
+    assert np.percentile(err, 99) <= BAR
+
+
+def _time_bins(acc, n):
+    """Eight bins by |a| rank: the fastest 0.05% in bin 0, fractions
+    growing geometrically to everything by bin 7."""
+    order = np.argsort(-np.linalg.norm(acc, axis=1), kind="stable")
+    frac = 0.0005 * (2000.0 ** (np.arange(8) / 7.0))
+    bins = np.empty(n, np.int32)
+    lo = 0
+    for b in range(8):
+        hi = n if b == 7 else int(round(frac[b] * n))
+        bins[order[lo:hi]] = b
+        lo = hi
+    return bins
+
+
+def test_c4_clustered_substeps_eager_and_graph_at_full_size():
+    n = 1 << 23
+    pos, _, m = ics.clustered_box(n, nhalo=256, seed=11)
+    soft = 1.0 / (40.0 * n ** (1.0 / 3.0))
+    say("c4: %d particles, prepare + regular step" % n)
+    e = _fp32(softening=soft, alpha_rel=0.003, fp32_queue=1)
+    e.set_particles(pos, m)
+    e.setup_domain()
+    e.domain_decomposition()
+    e.compute_acceleration(True)
+    e.domain_decomposition()
+    e.compute_acceleration(False)
+    st = e.get("pos", "acc", "last_acc_mag")
+    bins = _time_bins(st["acc"], n)
+    e.capture_walk()
+    g = np.random.Generator(np.random.Philox(13))
+    for b, want_frac in ((7, 1.0), (4, 0.038), (0, 0.0005)):
+        act = np.flatnonzero(bins <= b).astype(np.int32)
+        assert abs(len(act) / n - want_frac) < 0.1 * want_frac
+        e.set_active(act)
+        e.update_particles(st["pos"], acc=st["acc"],
+                           last_acc_mag=st["last_acc_mag"])
+        e.compute_acceleration(False)
+        eager = e.get("acc", "cost")
+        counts = e.interaction_counts()
+        e.set_active(act)
+        e.update_particles(st["pos"], acc=st["acc"],
+                           last_acc_mag=st["last_acc_mag"])
+        e.launch_walk()
+        graph = e.get("acc", "cost")
+        np.testing.assert_array_equal(graph["acc"], eager["acc"])
+        np.testing.assert_array_equal(graph["cost"], eager["cost"])
+        assert e.interaction_counts() == counts
+        pick = np.sort(g.choice(act, 256, replace=False))
+        ref = e.forcetest_exact(pick)
+        err = rel_err(eager["acc"][pick], ref)
+        say("c4: bins <= %d (%d active): median %.3g p99 %.3g" % (
+            b, len(act), np.median(err), np.percentile(err, 99)))
+        assert np.percentile(err, 99) <= BAR
+    e.close()
