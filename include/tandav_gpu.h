@@ -214,6 +214,21 @@ int tdv_peano_keys(tdv_context *ctx, size_t n, const double *x,
 int tdv_sort_keys(tdv_context *ctx, size_t n, const uint64_t *keys,
 		uint32_t *perm_out);
 
+/* tdv_sort_keys without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc); it has no counterpart in the reference.  keys
+ * (2n words, (lo, hi) per key) and perm_out (n entries) are DEVICE
+ * pointers; stream is a hipStream_t (NULL = the null stream) on the
+ * current device.  The same kernels as tdv_sort_keys work in a static
+ * per-device scratch array of about 37 MB (held until the process ends),
+ * so n may be at most TDV_NOALLOC_MAX_KEYS (larger n: TDV_ERR_ARG, nothing
+ * launched).  Returns after perm_out is written (it synchronises the
+ * stream); TDV_ERR_DUP_KEYS if two keys are equal.  Every call on a GPU
+ * shares that GPU's scratch: calls that overlap on one GPU race. */
+#define TDV_NOALLOC_MAX_KEYS (1u << 20)
+int tdv_sort_keys_noalloc(size_t n, const uint64_t *keys, uint32_t *perm_out,
+		void *stream);
+
 /* Ewald_Correction (Gravity/periodic.c:35-113) for n separations
  * dr[3n] -> f[3n]. */
 int tdv_ewald_correction(tdv_context *ctx, size_t n, const double *dr,

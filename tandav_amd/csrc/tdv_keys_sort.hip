@@ -16,6 +16,7 @@
 #include "tdv_device.h"
 
 #include <cstring>
+#include <mutex>
 
 #define RS_THREADS 256
 #define RS_ITEMS 16
@@ -332,7 +333,8 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(
 	}
 }
 
-/* the buffers a key sort works in */
+/* the buffers a key sort works in: a context's own, or the no-allocation
+ * entry point's static scratch */
 struct SortWork {
 	hipStream_t stream;
 	uint64_t *sk[2];             /* n keys each */
@@ -491,6 +493,100 @@ int tdv_sort_device(tdv_context *c, size_t n, const uint64_t *keys,
 		uint32_t *perm_out)
 {
 	return sort_keys_work(c, context_work(c, perm_out), n, keys, perm_out);
+}
+
+/* ------------------------------------------------------------------------ */
+/* the key sort without allocation: a static scratch array per device      */
+/* ------------------------------------------------------------------------ */
+
+/* byte offsets of the work buffers in the scratch array, for
+ * TDV_NOALLOC_MAX_KEYS keys (each 256-byte aligned) */
+#define NA_N ((size_t)TDV_NOALLOC_MAX_KEYS)
+#define NA_ALIGN(x) (((x) + 255) & ~(size_t)255)
+#define NA_HIST_CAP (((NA_N + RS_TILE - 1) / RS_TILE) * RS_BINS + 1)
+#define NA_PART_CAP (NA_HIST_CAP / SC_TILE + 2)
+#define NA_SK0 ((size_t)0)
+#define NA_SK1 (NA_SK0 + NA_ALIGN(NA_N * 8))
+#define NA_SV0 (NA_SK1 + NA_ALIGN(NA_N * 8))
+#define NA_SV1 (NA_SV0 + NA_ALIGN(NA_N * 4))
+#define NA_HI (NA_SV1 + NA_ALIGN(NA_N * 4))
+#define NA_PT (NA_HI + NA_ALIGN(NA_N * 8))
+#define NA_HIST (NA_PT + NA_ALIGN(NA_N * 4))
+#define NA_PART (NA_HIST + NA_ALIGN(NA_HIST_CAP * 4))
+#define NA_ERR (NA_PART + NA_ALIGN(NA_PART_CAP * 4))
+#define NA_BYTES (NA_ERR + 256)
+
+/* one copy per GPU, held until the process ends; every call on a GPU uses
+ * that GPU's copy, so calls that overlap on one GPU race */
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+#define NA_MAX_DEVICES 64
+static std::mutex na_mutex;
+static unsigned char *na_ptr[NA_MAX_DEVICES];
+
+/* the device address of this GPU's scratch (host code never passes the
+ * array's own name: on the host it is not the device address) */
+static int noalloc_scratch(int dev, unsigned char **out)
+{
+	std::lock_guard<std::mutex> lock(na_mutex);
+	if (!na_ptr[dev]) {
+		void *p = nullptr;
+		if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess
+				|| !p) {
+			(void)hipGetLastError();
+			return TDV_ERR_HIP;
+		}
+		na_ptr[dev] = static_cast<unsigned char *>(p);
+	}
+	*out = na_ptr[dev];
+	return 0;
+}
+
+extern "C" int tdv_sort_keys_noalloc(size_t n, const uint64_t *keys,
+		uint32_t *perm_out, void *stream)
+{
+	if (n > NA_N)
+		return TDV_ERR_ARG;
+	if (n == 0)
+		return 0;
+	if (!keys || !perm_out)
+		return TDV_ERR_ARG;
+	int dev = -1;
+	if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NA_MAX_DEVICES) {
+		(void)hipGetLastError();
+		return TDV_ERR_HIP;
+	}
+	unsigned char *base = nullptr;
+	TDV_TRY(noalloc_scratch(dev, &base));
+	SortWork w;
+	w.stream = static_cast<hipStream_t>(stream);
+	w.sk[0] = reinterpret_cast<uint64_t *>(base + NA_SK0);
+	w.sk[1] = reinterpret_cast<uint64_t *>(base + NA_SK1);
+	w.sv[0] = reinterpret_cast<uint32_t *>(base + NA_SV0);
+	w.sv[1] = reinterpret_cast<uint32_t *>(base + NA_SV1);
+	w.hi_tmp = reinterpret_cast<uint64_t *>(base + NA_HI);
+	w.p_tmp = reinterpret_cast<uint32_t *>(base + NA_PT);
+	w.hist = reinterpret_cast<uint32_t *>(base + NA_HIST);
+	w.hist_cap = NA_HIST_CAP;
+	w.scan_part = reinterpret_cast<uint32_t *>(base + NA_PART);
+	w.d_err = reinterpret_cast<unsigned *>(base + NA_ERR);
+	/* the previous call's flags stay in the array: clear them */
+	if (hipMemsetAsync(w.d_err, 0, sizeof(unsigned), w.stream) != hipSuccess) {
+		(void)hipGetLastError();
+		return TDV_ERR_HIP;
+	}
+	TDV_TRY(sort_keys_work(nullptr, w, n, keys, perm_out));
+	unsigned e = 0;
+	if (hipMemcpyAsync(&e, w.d_err, sizeof(e), hipMemcpyDeviceToHost, w.stream)
+			!= hipSuccess || hipStreamSynchronize(w.stream) != hipSuccess) {
+		(void)hipGetLastError();
+		return TDV_ERR_HIP;
+	}
+	if (e & TDV_EBIT_DUP)
+		return TDV_ERR_DUP_KEYS;
+	if (e & TDV_EBIT_TIES)
+		return TDV_ERR_TIES;
+	return 0;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -134,6 +134,7 @@ def load_library(path=LIB_PATH):
     L.tdv_peano_keys.argtypes = [_ctx, _sz, _dp, _dp, _dp, _dp,
                                  ctypes.c_double, ctypes.c_int, _vp]
     L.tdv_sort_keys.argtypes = [_ctx, _sz, _vp, _vp]
+    L.tdv_sort_keys_noalloc.argtypes = [_sz, _vp, _vp, _vp]
     L.tdv_ewald_correction.argtypes = [_ctx, _sz, _dp, _dp]
     L.tdv_get_ewald_table.argtypes = [_ctx, _dp]
     L.tdv_set_ewald_table.argtypes = [_ctx, _dp]
