@@ -29,8 +29,10 @@ enum {
 	TDV_ERR_ARG = 1,          /* bad argument / call order               */
 	TDV_ERR_HIP = 2,          /* HIP runtime error                        */
 	TDV_ERR_NOMEM = 3,        /* device allocation failed                 */
-	TDV_ERR_DUP_KEYS = 4,     /* two particles share a 128-bit PH key     */
-	TDV_ERR_KEY_DEPTH = 5,    /* particles closer than the key resolution */
+	TDV_ERR_DUP_KEYS = 4,     /* tdv_sort_keys: two keys are equal        */
+	TDV_ERR_KEY_DEPTH = 5,    /* not raised: particles closer than the key
+	                             resolution are dumped as the reference
+	                             does (Gravity/tree_build.c:337-342)      */
 	TDV_ERR_RANGE = 6,        /* a particle range left the arrays         */
 	TDV_ERR_TIES = 7          /* a run of equal 64-bit key prefixes too long */
 };
@@ -143,7 +145,10 @@ int tdv_domain_decomposition_targets(tdv_context *ctx, int ntarget,
 int tdv_set_domain(tdv_context *ctx, double size, const double origin[3]);
 
 /* Sort_Particles_By_Peano_Key (peano.c:20-48) with the domain given to
- * tdv_set_domain: keys, key sort, reorder of every column. */
+ * tdv_set_domain: keys, key sort, reorder of every column.  Particles with
+ * equal keys (within Domain.Size / 2^42 on every axis) keep their input
+ * order; the reference's unstable quicksort (sort.c:253-341) puts them in
+ * an order of its own.  The tree build dumps them as the reference does. */
 int tdv_sort_particles(tdv_context *ctx);
 
 /* D[].TNode (domain.h:40-52) from the caller's own decomposition
@@ -178,6 +183,22 @@ int tdv_capture_walk(tdv_context *ctx);
 /* Replay the captured walk (Compute_Acceleration on a regular step without
  * a rebuild). */
 int tdv_launch_walk(tdv_context *ctx);
+
+/* The whole regular step of a periodic FP32 run as one HIP graph:
+ * Domain_Decomposition (domain.c:67-145: PH keys, key sort, reorder, the
+ * level-2 top nodes), Compute_Acceleration on a regular step (accel.c:15-71:
+ * safe_last_accel, octree build, the FP32 walk with its Ewald correction)
+ * and, when kick or drift is nonzero, the KDK step of tdv_kick_drift with
+ * those factors.  It needs one eager step on the same particles first (the
+ * graph keeps the capacities of that step's node arrays and walk records);
+ * nothing runs at capture.  Engine-side; no counterpart in the reference. */
+int tdv_capture_step(tdv_context *ctx, double kick, double drift);
+/* Replay the captured step.  The state afterwards is the eager step's, bit
+ * for bit; stage time 5 holds the whole step.  TDV_ERR_TIES (more than 4096
+ * equal 64-bit key prefixes: the captured sort has no fallback) and
+ * TDV_ERR_RANGE (a tree larger than the captured arrays) ask for an eager
+ * step instead; a graph is dropped whenever its buffers may move. */
+int tdv_launch_step(tdv_context *ctx);
 
 /* Gravity_Forcetest (Gravity/forcetest.c:24-174): FP64 direct summation
  * over all particles (plus the tabulated Ewald correction of every pair on
@@ -383,8 +404,20 @@ int tdv_get_integer_times(tdv_context *ctx, uint32_t *it_drift,
  * particles: each kicks by half of its own bin from its own It_Kick_Pos,
  * v += Particle_Kick_Step(It_Kick_Pos, It_Kick_Pos + 2^Time_Bin / 2) * Acc,
  * and advances It_Kick_Pos; FP64, the factor per particle on the device
- * (comoving: the kick spline at both ends).  Timed as stage 7. */
-int tdv_kick_halfstep(tdv_context *ctx);
+ * (comoving: the kick spline at both ends).  With update_tree (a step
+ * without a domain update, !Sig.Domain_Update) the tree takes the kicks too,
+ * Gravity_Tree_Update_Kicks (Gravity/tree_update.c:12-57): every tree node
+ * on a kicked particle's path below its subtree root takes
+ * Dp += m dt a / Mass and the UPDATED bit, its top node the same in float
+ * (Level < 0 in tdv_get_top_nodes until the drift); each node's sum runs
+ * over its particles in particle order, as the reference's one-thread sums.
+ * Timed as stage 7. */
+int tdv_kick_halfstep(tdv_context *ctx, int update_tree);
+
+/* Gravity_Tree_Update_Drift (Gravity/tree_update.c:64-104): the UPDATED
+ * nodes move CoM += dt Dp and clear Dp and the flag; the kicked top nodes
+ * the same in float. */
+int tdv_tree_update_drift(tdv_context *ctx, double dt);
 
 /* Drift_To_Sync_Point (drift.c:12-64) for the active particles: each drifts
  * from its own It_Drift_Pos to it_next (Int_Time.Next),

@@ -89,6 +89,19 @@ void tdv_graph_drop(tdv_context *c)
 		(void)hipGraphDestroy(c->graph);
 	c->graph_exec = nullptr;
 	c->graph = nullptr;
+	if (c->step_exec)
+		(void)hipGraphExecDestroy(c->step_exec);
+	if (c->step_graph)
+		(void)hipGraphDestroy(c->step_graph);
+	c->step_exec = nullptr;
+	c->step_graph = nullptr;
+}
+
+/* stage events, left out of a captured step */
+static void rec(tdv_context *c, int e)
+{
+	if (!c->graph_mode)
+		(void)hipEventRecord(c->ev[e], c->stream);
 }
 
 /* the first and second derivatives of the Ewald correction at the table
@@ -131,6 +144,7 @@ static void free_particles(tdv_context *c)
 	dfree(c->id);
 	dfree(c->it_drift);
 	dfree(c->it_kick);
+	dfree(c->kick_dt);
 	dfree(c->pm);
 	dfree(c->active);
 	for (int b = 0; b < 2; b++) {
@@ -170,6 +184,7 @@ static int alloc_particles(tdv_context *c, size_t n)
 	TDV_TRY(dalloc(c, &c->id, n));
 	TDV_TRY(dalloc(c, &c->it_drift, n));
 	TDV_TRY(dalloc(c, &c->it_kick, n));
+	TDV_TRY(dalloc(c, &c->kick_dt, n));
 	TDV_TRY(dalloc(c, &c->pm, n));
 	TDV_TRY(dalloc(c, &c->active, n));
 	for (int b = 0; b < 2; b++) {
@@ -199,12 +214,14 @@ int tdv_ensure_nodes(tdv_context *c, size_t nodes)
 	dfree(c->node_b);
 	dfree(c->node_c);
 	dfree(c->node_up);
+	dfree(c->node_dp);
 	/* Max_Nodes grows by TREE_ENLARGEMENT_FACTOR (tree_build.c:6,162) */
 	size_t cap = (size_t)(1.2 * (double)nodes) + 1024;
 	TDV_TRY(dalloc(c, &c->node_a, cap));
 	TDV_TRY(dalloc(c, &c->node_b, cap));
 	TDV_TRY(dalloc(c, &c->node_c, cap));
 	TDV_TRY(dalloc(c, &c->node_up, cap));
+	TDV_TRY(dalloc(c, &c->node_dp, cap));
 	c->node_cap = cap;
 	return 0;
 }
@@ -221,6 +238,8 @@ static void free_top(tdv_context *c)
 	dfree(t.pos);
 	dfree(t.mass);
 	dfree(t.com);
+	dfree(t.dp);
+	dfree(t.kicked);
 	dfree(t.ntop);
 	dfree(t.nnodes);
 	c->top_cap = 0;
@@ -241,6 +260,8 @@ static int alloc_top(tdv_context *c, int cap)
 	TDV_TRY(dalloc(c, &t.pos, 3 * (size_t)cap));
 	TDV_TRY(dalloc(c, &t.mass, cap));
 	TDV_TRY(dalloc(c, &t.com, 3 * (size_t)cap));
+	TDV_TRY(dalloc(c, &t.dp, 3 * (size_t)cap));
+	TDV_TRY(dalloc(c, &t.kicked, cap));
 	TDV_TRY(dalloc(c, &t.ntop, 1));
 	TDV_TRY(dalloc(c, &t.nnodes, 1));
 	TDV_HIP(c, hipMemset(t.ntop, 0, sizeof(int)));
@@ -531,17 +552,34 @@ __global__ void k_iota(int n, int *__restrict__ a)
  * keys, sort, reorder of every column.  The active list's indices belong to
  * the old order, so it is reset to every particle (a sync point, as at every
  * domain decomposition of the reference); tdv_set_active narrows it. */
+__global__ void k_set_int(int *p, int v)
+{
+	if (threadIdx.x == 0)
+		*p = v;
+}
+
+__global__ void k_clear_err_bits(unsigned *err, unsigned bits)
+{
+	if (threadIdx.x == 0)
+		*err &= ~bits;
+}
+
 static int keys_sort_reorder(tdv_context *c)
 {
 	const size_t n = c->n;
 	if (!n)
 		return tdv_fail(c, TDV_ERR_ARG, "no particles");
-	(void)hipEventRecord(c->ev[EV_KEYS0], c->stream);
+	rec(c, EV_KEYS0);
 	tdv_launch_keys(c, n, c->pos[0], c->pos[1], c->pos[2], c->dom_origin,
 			c->dom_size, c->key_tmp, 0);
-	(void)hipEventRecord(c->ev[EV_KEYS1], c->stream);
+	rec(c, EV_KEYS1);
 	TDV_TRY(tdv_sort_device(c, n, c->key_tmp, c->perm));
-	(void)hipEventRecord(c->ev[EV_SORT1], c->stream);
+	/* equal keys (particles within Domain.Size / 2^42 on every axis) stay in
+	 * their input order: the reference's quicksort leaves them in an order
+	 * of its own (sort.c:253-341), the build dumps them all the same */
+	hipLaunchKernelGGL(k_clear_err_bits, dim3(1), dim3(64), 0, c->stream,
+			c->d_err, TDV_EBIT_DUP);
+	rec(c, EV_SORT1);
 	/* reorder_collisionless_particles (peano.c:50-82) as gathers */
 	tdv_launch_gather_key(c, n, c->perm, c->key_tmp, c->key);
 	double **cols[] = { &c->pos[0], &c->pos[1], &c->pos[2], &c->vel[0],
@@ -550,6 +588,13 @@ static int keys_sort_reorder(tdv_context *c)
 		&c->cost };
 	for (double **col : cols) {
 		tdv_launch_gather_f64(c, n, c->perm, *col, c->tmp_f64);
+		if (c->graph_mode) {
+			/* a captured step keeps every column where it is, so each
+			 * replay finds its buffers: copy back instead of swapping */
+			TDV_HIP(c, hipMemcpyAsync(*col, c->tmp_f64, n * sizeof(double),
+					hipMemcpyDeviceToDevice, c->stream));
+			continue;
+		}
 		double *t = *col;
 		*col = c->tmp_f64;
 		c->tmp_f64 = t;
@@ -563,6 +608,19 @@ static int keys_sort_reorder(tdv_context *c)
 				hipMemcpyDeviceToDevice, c->stream));
 	}
 	tdv_launch_pack_pm(c);
+	if (c->graph_mode) {
+		/* every particle active after the sort, the count on the device */
+		hipLaunchKernelGGL(k_iota, dim3(tdv_blocks(n, 256)), dim3(256), 0,
+				c->stream, (int)n, c->active);
+		hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream,
+				c->d_nactive, (int)n);
+		TDV_HIP(c, hipGetLastError());
+		c->nactive = (int)n;
+		c->have_domain = 1;
+		c->have_tree = 0;
+		c->have_top = 0;
+		return 0;
+	}
 	if ((size_t)c->nactive != n) {
 		hipLaunchKernelGGL(k_iota, dim3(tdv_blocks(n, 256)), dim3(256), 0,
 				c->stream, (int)n, c->active);
@@ -821,6 +879,121 @@ int tdv_launch_walk(tdv_context *c)
 	return tdv_check_err_flags(c);
 }
 
+/* The whole regular step of a periodic FP32 run as one HIP graph: the
+ * decomposition (keys, sort, reorder, level-2 top nodes), safe_last_accel,
+ * the build, the FP32 walk's records, moments and walk, and the kick-drift
+ * of one global step.  Captured with the capacities of the last eager step
+ * (node arrays, walk records), every count read on the device, columns
+ * copied back after the reorder instead of swapped, so a replay finds its
+ * buffers; nothing runs at capture. */
+int tdv_capture_step(tdv_context *c, double kick, double drift)
+{
+	if (!c || c->par.precision != TDV_WALK_FP32 || !c->par.periodic)
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_capture_step: the FP32 walk of a "
+				"periodic box only");
+	if (!std::isfinite(kick) || !std::isfinite(drift))
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_capture_step: factors not "
+				"finite");
+	const size_t long_need = (c->n / (TDV_LONG_CHAIN + 1) + 1) * (TDV_NT + 2);
+	if (!c->have_tree || !c->p32 || !c->ewald16_ok || !c->partial
+			|| c->n32_cap < c->node_cap || c->top32_cap < c->top_cap
+			|| c->top_cap < 64 || c->long_cap < long_need
+			|| (c->par.fp32_order != 1 && !c->lvl_nodes))
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_capture_step: run one eager "
+				"step (decomposition, FP32 walk) on these particles first");
+	tdv_graph_drop(c);
+	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	/* host-side state the captured calls set; nothing has run yet */
+	const int nact0 = c->nactive, dom0 = c->have_domain, top0 = c->have_top,
+			tree0 = c->have_tree, lvl0 = c->lvl_ok;
+	TDV_HIP(c, hipStreamBeginCapture(c->stream,
+			hipStreamCaptureModeThreadLocal));
+	c->graph_mode = 1;
+	c->capturing = 1;
+	int r = computational_domain(c);
+	if (!r)
+		r = keys_sort_reorder(c);
+	if (!r)
+		r = tdv_build_top_nodes(c);
+	if (!r) {
+		c->have_top = 1;
+		tdv_launch_safe_last_accel_dev(c);
+		r = tdv_build_tree(c);
+	}
+	if (!r)
+		r = tdv_walk32(c, 0);
+	if (!r)
+		tdv_launch_kick_drift(c, kick, drift);
+	c->graph_mode = 0;
+	c->capturing = 0;
+	hipGraph_t g = nullptr;
+	const hipError_t e = hipStreamEndCapture(c->stream, &g);
+	c->nactive = nact0;
+	c->have_domain = dom0;
+	c->have_top = top0;
+	c->have_tree = tree0;
+	c->lvl_ok = lvl0;
+	if (r) {
+		if (g)
+			(void)hipGraphDestroy(g);
+		return r;
+	}
+	TDV_TRY(tdv_check(c, e, "hipStreamEndCapture"));
+	c->step_graph = g;
+	TDV_TRY(tdv_check(c, hipGraphInstantiate(&c->step_exec, g, nullptr,
+			nullptr, 0), "hipGraphInstantiate"));
+	c->step_kick = kick;
+	c->step_drift = drift;
+	return 0;
+}
+
+/* Replay the captured step; the counts and error flags are read after it. */
+int tdv_launch_step(tdv_context *c)
+{
+	if (!c || !c->step_exec)
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_launch_step: nothing captured");
+	(void)hipEventRecord(c->ev[EV_KEYS0], c->stream);
+	TDV_HIP(c, hipGraphLaunch(c->step_exec, c->stream));
+	(void)hipEventRecord(c->ev[EV_KICK1], c->stream);
+	int h[2];
+	TDV_HIP(c, hipMemcpyAsync(&h[0], c->top.ntop, sizeof(int),
+			hipMemcpyDeviceToHost, c->stream));
+	TDV_HIP(c, hipMemcpyAsync(&h[1], c->top.nnodes, sizeof(int),
+			hipMemcpyDeviceToHost, c->stream));
+	TDV_HIP(c, hipMemcpyAsync(c->count, c->d_count, sizeof(c->count),
+			hipMemcpyDeviceToHost, c->stream));
+	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	float ms = 0.f;
+	(void)hipEventElapsedTime(&ms, c->ev[EV_KEYS0], c->ev[EV_KICK1]);
+	for (int k = 0; k < 8; k++)
+		c->stage_ms[k] = 0.f;
+	c->stage_ms[5] = ms;                 /* the whole step */
+	c->ntop = h[0];
+	c->nactive = (int)c->n;
+	c->have_domain = 1;
+	c->have_top = 1;
+	c->have_tree = h[1] >= 0;
+	c->lvl_ok = h[1] >= 0;
+	c->nnodes = h[1] >= 0 ? (size_t)h[1] : 0;
+	unsigned err = 0;
+	TDV_HIP(c, hipMemcpy(&err, c->d_err, sizeof(err), hipMemcpyDeviceToHost));
+	if (err & TDV_EBIT_TIES) {
+		TDV_HIP(c, hipMemset(c->d_err, 0, sizeof(unsigned)));
+		c->have_tree = 0;
+		return tdv_fail(c, TDV_ERR_TIES, "tdv_launch_step: a run of more "
+				"than 4096 equal 64-bit key prefixes needs the sort's "
+				"fallback, which a captured step does not take: run the step "
+				"eagerly");
+	}
+	if (h[1] < 0) {
+		TDV_HIP(c, hipMemset(c->d_err, 0, sizeof(unsigned)));
+		return tdv_fail(c, TDV_ERR_RANGE, "tdv_launch_step: the tree needs "
+				"more nodes than the captured step holds: run the step "
+				"eagerly and capture again");
+	}
+	return tdv_check_err_flags(c);
+}
+
 enum { FT_TABLE, FT_EXACT_SPLIT, FT_EXACT_PAIRS };
 
 static int forcetest(tdv_context *c, int nt, const int *idx, double *acc_out,
@@ -1074,7 +1247,8 @@ int tdv_get_top_nodes(tdv_context *c, void *out, int max, int *ntop)
 		return tdv_fail(c, TDV_ERR_ARG, "top node buffer too small");
 	std::vector<int> first(n), npart(n), level(n), target(n);
 	std::vector<uint64_t> key(n);
-	std::vector<float> pos(3 * n), mass(n), com(3 * n);
+	std::vector<float> pos(3 * n), mass(n), com(3 * n), tdp(3 * n);
+	std::vector<int> kicked(n);
 	auto cp = [&](void *h, const void *d, size_t b) {
 		return b ? tdv_check(c, hipMemcpyAsync(h, d, b, hipMemcpyDeviceToHost,
 				c->stream), "top copy") : 0;
@@ -1087,6 +1261,8 @@ int tdv_get_top_nodes(tdv_context *c, void *out, int max, int *ntop)
 	TDV_TRY(cp(pos.data(), c->top.pos, n * 12));
 	TDV_TRY(cp(mass.data(), c->top.mass, n * 4));
 	TDV_TRY(cp(com.data(), c->top.com, n * 12));
+	TDV_TRY(cp(tdp.data(), c->top.dp, n * 12));
+	TDV_TRY(cp(kicked.data(), c->top.kicked, n * 4));
 	TDV_HIP(c, hipStreamSynchronize(c->stream));
 	TopRecord *rec = static_cast<TopRecord *>(out);
 	for (int t = 0; t < n; t++) {
@@ -1094,12 +1270,14 @@ int tdv_get_top_nodes(tdv_context *c, void *out, int max, int *ntop)
 		memset(&r, 0, sizeof(r));
 		r.Key = key[t];
 		r.Target = c->have_tree ? target[t] : -1;
-		r.Level = level[t];
+		/* kicked since the last drift: Level < 0 (tree_update.c:46-47) */
+		r.Level = c->have_tree && kicked[t] ? -level[t] : level[t];
 		r.First_Part = first[t];
 		r.Npart = npart[t];
 		for (int d = 0; d < 3; d++) {
 			r.Pos[d] = pos[3 * t + d];
 			r.CoM[d] = c->have_tree ? com[3 * t + d] : 0.f;
+			r.Dp[d] = c->have_tree ? tdp[3 * t + d] : 0.f;
 		}
 		r.Mass = c->have_tree ? mass[t] : 0.f;
 		rec[t] = r;
@@ -1123,6 +1301,9 @@ int tdv_get_tree(tdv_context *c, void *out, size_t max, size_t *nnodes)
 	std::vector<double4> a(n), b(n);
 	std::vector<int4> cc(n);
 	std::vector<int> up(n);
+	std::vector<double4> dp(n);
+	TDV_HIP(c, hipMemcpyAsync(dp.data(), c->node_dp, n * 32,
+			hipMemcpyDeviceToHost, c->stream));
 	TDV_HIP(c, hipMemcpyAsync(a.data(), c->node_a, n * 32,
 			hipMemcpyDeviceToHost, c->stream));
 	TDV_HIP(c, hipMemcpyAsync(b.data(), c->node_b, n * 32,
@@ -1147,6 +1328,9 @@ int tdv_get_tree(tdv_context *c, void *out, size_t max, size_t *nnodes)
 		r.CoM[0] = a[j].x;
 		r.CoM[1] = a[j].y;
 		r.CoM[2] = a[j].z;
+		r.Dp[0] = dp[j].x;
+		r.Dp[1] = dp[j].y;
+		r.Dp[2] = dp[j].z;
 		rec[j] = r;
 	}
 	return 0;

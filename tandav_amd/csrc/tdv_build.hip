@@ -83,11 +83,12 @@ __global__ void __launch_bounds__(64) k_top_nodes(size_t n,
 
 /* per particle: the top node holding it (top nodes are contiguous particle
  * ranges in descending key order, i.e. descending first particle) */
-__global__ void k_assign_top(size_t n, TopNodes t, int ntop)
+__global__ void k_assign_top(size_t n, TopNodes t)
 {
 	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
+	const int ntop = *t.ntop;
 	int lo = 0, hi = ntop - 1;          /* first[] decreases with index */
 	while (lo < hi) {
 		const int mid = (lo + hi) >> 1;
@@ -102,7 +103,7 @@ __global__ void k_assign_top(size_t n, TopNodes t, int ntop)
 int tdv_assign_top(tdv_context *c)
 {
 	hipLaunchKernelGGL(k_assign_top, dim3(tdv_blocks(c->n, 256)), dim3(256), 0,
-			c->stream, c->n, c->top, c->ntop);
+			c->stream, c->n, c->top);
 	return tdv_check(c, hipGetLastError(), "k_assign_top");
 }
 
@@ -112,15 +113,25 @@ int tdv_build_top_nodes(tdv_context *c)
 			c->key, c->pos[0], c->pos[1], c->pos[2], c->dom_origin[0],
 			c->dom_origin[1], c->dom_origin[2], c->dom_size, c->top);
 	TDV_HIP(c, hipGetLastError());
-	TDV_HIP(c, hipMemcpyAsync(&c->ntop, c->top.ntop, sizeof(int),
-			hipMemcpyDeviceToHost, c->stream));
-	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	if (!c->graph_mode) {            /* a captured step keeps it on the device */
+		TDV_HIP(c, hipMemcpyAsync(&c->ntop, c->top.ntop, sizeof(int),
+				hipMemcpyDeviceToHost, c->stream));
+		TDV_HIP(c, hipStreamSynchronize(c->stream));
+	}
 	return tdv_assign_top(c);
 }
 
 /* ------------------------------------------------------------------------ */
 /* tree                                                                      */
 /* ------------------------------------------------------------------------ */
+
+/* the triplet a node's Bitfield holds (tree.h:25,58): the reversed key's
+ * (key_fragment, tree_build.c:580-585), whose level 42 has lost its top
+ * bit (peano.c:254-275) */
+__device__ __forceinline__ unsigned bitfield_triplet(Key2 k, int lv)
+{
+	return key_triplet(k, lv) & (lv >= TDV_NT ? 3u : 7u);
+}
 
 struct TopRange {
 	int t, first, last;     /* top node, its particle range */
@@ -179,12 +190,19 @@ __global__ void __launch_bounds__(256) k_tree_marks(size_t n,
 	if (!r.tree || i == r.first)
 		return;
 	const int dp = dprev_of(key, i, r.first, r.level);
-	if (dp >= TDV_NT) {          /* below key resolution: not supported */
-		atomicOr(err, TDV_EBIT_DEPTH);
+	/* below the key resolution (its key matches the previous one down to
+	 * level 41, or entirely): the reference dumps the particle into the
+	 * level-41 node it reached, with no node of its own and no collapse
+	 * event, leaving last_key and last_parent as they were
+	 * (tree_build.c:337-342) */
+	if (dp >= TDV_NT)
 		return;
-	}
-	const int dpp = dprev_of(key, i - 1, r.first, r.level);
-	const bool newb = (i >= r.first + 2) && dp < dpp;
+	/* the previous particle that was inserted: dumped ones are skipped */
+	int h = i - 1;
+	while (h > r.first && dprev_of(key, h, r.first, r.level) >= TDV_NT)
+		h--;
+	const int dpp = dprev_of(key, h, r.first, r.level);
+	const bool newb = (h >= r.first + 1) && dp < dpp;
 	if (!(newb || i == r.last))
 		return;
 	int lk = -1, cnt = 0;
@@ -222,6 +240,11 @@ __global__ void __launch_bounds__(256) k_tree_counts(size_t n,
 		return;
 	}
 	const int lo = dprev_of(key, i, r.first, r.level);
+	if (lo >= TDV_NT) {              /* dumped: no node of its own */
+		cnt[i] = 0;
+		lohi[i] = 0;
+		return;
+	}
 	const int dn = (i == r.last) ? -1 : div_level(load_key(key, i),
 			load_key(key, i + 1));
 	const int lf = lo > dn ? lo : dn;
@@ -233,21 +256,50 @@ __global__ void __launch_bounds__(256) k_tree_counts(size_t n,
 	lohi[i] = lo | (hi << 8) | ((ml <= lf) ? (1 << 16) : 0);
 }
 
+/* the node ranges of the top nodes' subtrees; more nodes than `cap` (a
+ * captured step's arrays) set the node count to -1, which every later
+ * kernel of the step reads as "do nothing", and raise TDV_EBIT_RANGE */
 __global__ void __launch_bounds__(64) k_tree_targets(TopNodes T, int vs,
-		const uint32_t *__restrict__ off, int ntop)
+		const uint32_t *__restrict__ off, long long cap, unsigned *err)
 {
 	if (threadIdx.x != 0)
 		return;
-	int run = 0;
+	const int ntop = *T.ntop;
+	long long run = 0;
 	for (int t = 0; t < ntop; t++) {
-		int sz = 0;
+		long long sz = 0;
 		if (T.npart[t] > vs)
-			sz = (int)(off[T.first[t] + T.npart[t]] - off[T.first[t]]) + 1;
-		T.target[t] = run;           /* reserve_tree_memory(nNeeded) */
-		T.term[t] = run + sz - 1;
+			sz = (long long)(off[T.first[t] + T.npart[t]]
+					- off[T.first[t]]) + 1;
+		T.target[t] = (int)run;      /* reserve_tree_memory(nNeeded) */
+		T.term[t] = (int)(run + sz - 1);
 		run += sz;
 	}
-	*T.nnodes = run;
+	if (run > cap) {
+		atomicOr(err, TDV_EBIT_RANGE);
+		*T.nnodes = -1;
+	} else {
+		*T.nnodes = (int)run;
+	}
+}
+
+/* node records cleared before the build fills them (the count on the
+ * device) */
+__global__ void k_clear_nodes(size_t bound, const TopNodes T,
+		int4 *__restrict__ nc, double4 *__restrict__ dp)
+{
+	const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (j < bound && (long long)j < (long long)*T.nnodes) {
+		nc[j] = make_int4(0, 0, 0, 0);
+		dp[j] = make_double4(0, 0, 0, 0);
+	}
+}
+
+__global__ void k_fill_u32_val(size_t n, uint32_t *__restrict__ p, uint32_t v)
+{
+	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		p[i] = v;
 }
 
 __device__ __forceinline__ int node_index(const TopNodes &T, int t, int first,
@@ -272,7 +324,7 @@ __global__ void __launch_bounds__(256) k_tree_nodes(size_t n,
 	if (gi >= n)
 		return;
 	const int i = (int)gi;
-	if (cnt[i] == 0)
+	if (cnt[i] == 0 || *T.nnodes < 0)
 		return;
 	const TopRange r = top_of(key, gi, T, vs);
 	const int lo = lohi[i] & 0xFF, hi = (lohi[i] >> 8) & 0xFF;
@@ -325,7 +377,7 @@ __global__ void __launch_bounds__(256) k_tree_nodes(size_t n,
 			for (int lv = lowest; lv <= cur; lv++) {
 				const int j = base + (lv - lo);
 				const bool leaf = lv == hi;
-				int bits = lv | ((int)key_triplet(ki, lv) << 6)
+				int bits = lv | ((int)bitfield_triplet(ki, lv) << 6)
 						| (1 << TDV_BIT_LOCAL);
 				const bool root = j == T.target[r.t];
 				if (root)
@@ -340,6 +392,9 @@ __global__ void __launch_bounds__(256) k_tree_nodes(size_t n,
 					up[j] = 1;
 				if (leaf)
 					for (int q = i; q < kn; q++)
+						tree_parent[q] = j;
+				else if (lv == TDV_NT - 1)   /* the dumped particles' node */
+					for (int q = i + 1; q < kn; q++)
 						tree_parent[q] = j;
 			}
 			cur = lowest - 1;
@@ -410,6 +465,8 @@ __global__ void __launch_bounds__(64) k_tree_nodes_long(
 	__shared__ int shdn[64];
 	const int lane = threadIdx.x;
 	/* an overflow is flagged by k_tree_nodes; read no further than the list */
+	if (*T.nnodes < 0)
+		return;
 	const unsigned nl = *long_count;
 	const int nlong = (int)(nl < long_cap ? nl : long_cap);
 	for (;;) {
@@ -521,7 +578,7 @@ __global__ void __launch_bounds__(64) k_tree_nodes_long(
 				for (int lv = lowest + lane; lv <= cur; lv += 64) {
 					const int j = base + (lv - lo);
 					const bool leaf = lv == hi;
-					int bits = lv | ((int)key_triplet(ki, lv) << 6)
+					int bits = lv | ((int)bitfield_triplet(ki, lv) << 6)
 							| (1 << TDV_BIT_LOCAL);
 					const bool root = j == T.target[r.t];
 					if (root)
@@ -533,6 +590,9 @@ __global__ void __launch_bounds__(64) k_tree_nodes_long(
 						up[j] = r.t;
 					else if (lv > lo)
 						up[j] = 1;
+					if (!leaf && lv == TDV_NT - 1)   /* dumped particles */
+						for (int q = i + 1; q < kn; q++)
+							tree_parent[q] = j;
 				}
 				if (cur == hi)            /* the leaf: its particles */
 					for (int q = i + lane; q < kn; q += 64)
@@ -558,10 +618,10 @@ __global__ void k_tree_long_reset(unsigned *__restrict__ long_count,
 __global__ void k_tree_small_top(const double *__restrict__ x,
 		const double *__restrict__ y, const double *__restrict__ z,
 		const double *__restrict__ m, TopNodes T, int vs,
-		const double4 *__restrict__ na, int *__restrict__ tree_parent, int ntop)
+		const double4 *__restrict__ na, int *__restrict__ tree_parent)
 {
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= ntop)
+	if (t >= *T.ntop || *T.nnodes < 0)
 		return;
 	const int f = T.first[t], np = T.npart[t];
 	double cx, cy, cz, mm;
@@ -589,14 +649,16 @@ __global__ void k_tree_small_top(const double *__restrict__ x,
 	T.com[3 * t] = (float)cx;
 	T.com[3 * t + 1] = (float)cy;
 	T.com[3 * t + 2] = (float)cz;
+	for (int d = 0; d < 3; d++)          /* no tree kicks yet */
+		T.dp[3 * t + d] = 0.f;
+	T.kicked[t] = 0;
 }
 
 __global__ void k_tree_terms(TopNodes T, int vs, double4 *__restrict__ na,
-		double4 *__restrict__ nb, int4 *__restrict__ nc, int *__restrict__ up,
-		int ntop)
+		double4 *__restrict__ nb, int4 *__restrict__ nc, int *__restrict__ up)
 {
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= ntop || T.npart[t] <= vs)
+	if (t >= *T.ntop || *T.nnodes < 0 || T.npart[t] <= vs)
 		return;
 	const int j = T.term[t];
 	na[j] = make_double4(0, 0, 0, 1);   /* terminator: Mass = 1, TOP bit */
@@ -605,15 +667,16 @@ __global__ void k_tree_terms(TopNodes T, int vs, double4 *__restrict__ na,
 	up[j] = 0;
 }
 
-__global__ void __launch_bounds__(256) k_tree_centres(size_t nnodes,
+__global__ void __launch_bounds__(256) k_tree_centres(size_t bound,
 		const double *__restrict__ x, const double *__restrict__ y,
 		const double *__restrict__ z, TopNodes T, double dom_size,
 		const int4 *__restrict__ nc, const int *__restrict__ up,
-		double4 *__restrict__ nb, size_t npart, unsigned *err, int ntop)
+		double4 *__restrict__ nb, size_t npart, unsigned *err)
 {
 	const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (j >= nnodes)
+	if (j >= bound || (long long)j >= (long long)*T.nnodes)
 		return;
+	const int ntop = *T.ntop;
 	const int4 me = nc[j];
 	if (me.w < 0)
 		return;                          /* terminator */
@@ -655,13 +718,18 @@ __global__ void __launch_bounds__(256) k_tree_centres(size_t nnodes,
 	nb[j] = make_double4(cx, cy, cz, dom_size / (double)(1ull << lv));
 }
 
+/* The build.  Eagerly it reads the node count back to size the node
+ * arrays; in a captured step (c->graph_mode) the arrays keep the capacity
+ * of the last eager build and every kernel takes the counts from device
+ * memory (a count beyond the capacity is flagged, see k_tree_targets). */
 int tdv_build_tree(tdv_context *c)
 {
 	const size_t n = c->n;
 	const int vs = c->par.vector_size;
+	const bool graph = c->graph_mode != 0;
 	c->lvl_ok = 0;                       /* the FP32 walk's level lists */
-	TDV_HIP(c, hipMemsetAsync(c->minpack, 0xFF, n * sizeof(uint32_t),
-			c->stream));
+	hipLaunchKernelGGL(k_fill_u32_val, dim3(tdv_blocks(n, 256)), dim3(256), 0,
+			c->stream, n, c->minpack, 0xFFFFFFFFu);
 	hipLaunchKernelGGL(k_tree_marks, dim3(tdv_blocks(n, 256)), dim3(256), 0,
 			c->stream, n, c->key, c->top, vs, c->minpack, c->d_err);
 	hipLaunchKernelGGL(k_tree_counts, dim3(tdv_blocks(n, 256)), dim3(256), 0,
@@ -669,21 +737,28 @@ int tdv_build_tree(tdv_context *c)
 			reinterpret_cast<uint32_t *>(c->cnt), c->lohi);
 	TDV_TRY(tdv_scan_u32(c, reinterpret_cast<uint32_t *>(c->cnt), c->off, n));
 	hipLaunchKernelGGL(k_tree_targets, dim3(1), dim3(64), 0, c->stream, c->top,
-			vs, c->off, c->ntop);
+			vs, c->off, graph ? (long long)c->node_cap : (1ll << 62),
+			c->d_err);
 	TDV_HIP(c, hipGetLastError());
-	int nn = 0;
-	TDV_HIP(c, hipMemcpyAsync(&nn, c->top.nnodes, sizeof(int),
-			hipMemcpyDeviceToHost, c->stream));
-	TDV_HIP(c, hipStreamSynchronize(c->stream));
-	TDV_TRY(tdv_check_err_flags(c));
-	TDV_TRY(tdv_ensure_nodes(c, (size_t)nn));
-	c->nnodes = (size_t)nn;
-	if (nn > 0)
-		TDV_HIP(c, hipMemsetAsync(c->node_c, 0, (size_t)nn * sizeof(int4),
-				c->stream));
+	size_t bound = c->node_cap;          /* nodes the kernels may touch */
+	if (!graph) {
+		int nn = 0;
+		TDV_HIP(c, hipMemcpyAsync(&nn, c->top.nnodes, sizeof(int),
+				hipMemcpyDeviceToHost, c->stream));
+		TDV_HIP(c, hipStreamSynchronize(c->stream));
+		TDV_TRY(tdv_check_err_flags(c));
+		TDV_TRY(tdv_ensure_nodes(c, (size_t)nn));
+		c->nnodes = (size_t)nn;
+		bound = (size_t)nn;
+	}
+	if (bound > 0)
+		hipLaunchKernelGGL(k_clear_nodes, dim3(tdv_blocks(bound, 256)),
+				dim3(256), 0, c->stream, bound, c->top, c->node_c, c->node_dp);
 	/* at most n / (TDV_LONG_CHAIN + 1) disjoint long cells per level */
 	const size_t long_need = (n / (TDV_LONG_CHAIN + 1) + 1) * (TDV_NT + 2);
 	if (c->long_cap < long_need) {
+		if (graph)
+			return tdv_fail(c, TDV_ERR_ARG, "capture: build lists missing");
 		if (c->long_list)
 			(void)hipFree(c->long_list);
 		c->long_list = nullptr;
@@ -703,17 +778,18 @@ int tdv_build_tree(tdv_context *c)
 			c->top, vs, reinterpret_cast<uint32_t *>(c->cnt), c->off, c->lohi,
 			c->node_a, c->node_c, c->node_up, c->tree_parent, c->long_list,
 			c->long_count, (unsigned)c->long_cap, c->long_count + 1);
-	hipLaunchKernelGGL(k_tree_terms, dim3(tdv_blocks((size_t)c->ntop, 64)),
+	const int tgrid = graph ? c->top_cap : c->ntop;
+	hipLaunchKernelGGL(k_tree_terms, dim3(tdv_blocks((size_t)tgrid, 64)),
 			dim3(64), 0, c->stream, c->top, vs, c->node_a, c->node_b, c->node_c,
-			c->node_up, c->ntop);
-	hipLaunchKernelGGL(k_tree_small_top, dim3(tdv_blocks((size_t)c->ntop, 64)),
+			c->node_up);
+	hipLaunchKernelGGL(k_tree_small_top, dim3(tdv_blocks((size_t)tgrid, 64)),
 			dim3(64), 0, c->stream, c->pos[0], c->pos[1], c->pos[2], c->mass,
-			c->top, vs, c->node_a, c->tree_parent, c->ntop);
-	if (nn > 0)
-		hipLaunchKernelGGL(k_tree_centres, dim3(tdv_blocks((size_t)nn, 256)),
-				dim3(256), 0, c->stream, (size_t)nn, c->pos[0], c->pos[1],
+			c->top, vs, c->node_a, c->tree_parent);
+	if (bound > 0)
+		hipLaunchKernelGGL(k_tree_centres, dim3(tdv_blocks(bound, 256)),
+				dim3(256), 0, c->stream, bound, c->pos[0], c->pos[1],
 				c->pos[2], c->top, c->dom_size, c->node_c, c->node_up,
-				c->node_b, n, c->d_err, c->ntop);
+				c->node_b, n, c->d_err);
 	TDV_HIP(c, hipGetLastError());
 	c->have_tree = 1;
 	return 0;

@@ -191,7 +191,8 @@ __global__ void k_kick_bins(int nact, const int *__restrict__ active,
 		const int *__restrict__ bin, uint32_t *__restrict__ it_kick,
 		const double *__restrict__ ax, const double *__restrict__ ay,
 		const double *__restrict__ az, double *__restrict__ vx,
-		double *__restrict__ vy, double *__restrict__ vz, TimelineDev t)
+		double *__restrict__ vy, double *__restrict__ vz,
+		double *__restrict__ kick_dt, TimelineDev t)
 {
 	const int g = blockIdx.x * blockDim.x + threadIdx.x;
 	if (g >= nact)
@@ -204,6 +205,121 @@ __global__ void k_kick_bins(int nact, const int *__restrict__ active,
 	vy[i] = vy[i] + dt * ay[i];
 	vz[i] = vz[i] + dt * az[i];
 	it_kick[i] = i1;
+	kick_dt[i] = dt;
+}
+
+/* lower bound of v in the sorted active list */
+__device__ __forceinline__ int active_lower(const int *__restrict__ a, int n,
+		int v)
+{
+	int lo = 0, hi = n;
+	while (lo < hi) {
+		const int mid = (lo + hi) >> 1;
+		if (a[mid] < v)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	return lo;
+}
+
+/* Gravity_Tree_Update_Kicks (tree_update.c:12-57) for every active
+ * particle: each tree node on the particle's path below its subtree root
+ * (the nodes whose particle range holds it) takes Dp += m dt a / M, marked
+ * UPDATED; the node's particles in particle order, so the sums are the
+ * reference's one-thread sums. */
+__global__ void k_tree_kicks(size_t nn, int4 *__restrict__ nc,
+		const double4 *__restrict__ na, double4 *__restrict__ dpn,
+		int nact, const int *__restrict__ active,
+		const double *__restrict__ m, const double *__restrict__ dtk,
+		const double *__restrict__ ax, const double *__restrict__ ay,
+		const double *__restrict__ az)
+{
+	const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= nn)
+		return;
+	const int4 c = nc[j];
+	if (c.w < 0 || ((c.z >> TDV_BIT_TOP) & 1))
+		return;                          /* terminator, subtree root */
+	const int lo = active_lower(active, nact, c.w);
+	const int hi = active_lower(active, nact, c.w + c.y);
+	if (lo >= hi)
+		return;
+	const double M = na[j].w;
+	double4 d = dpn[j];
+	for (int g = lo; g < hi; g++) {
+		const int i = active[g];
+		const double mdt = m[i] * dtk[i];
+		d.x = d.x + mdt * ax[i] / M;
+		d.y = d.y + mdt * ay[i] / M;
+		d.z = d.z + mdt * az[i] / M;
+	}
+	dpn[j] = d;
+	nc[j].z = c.z | (1 << TDV_BIT_UPDATED);   /* only this node's own */
+}
+
+/* the same for the top nodes (tree_update.c:43-55), in float: Dp is a
+ * float in D[].TNode, and the node is marked kicked */
+__global__ void k_top_kicks(const TopNodes T, int nact,
+		const int *__restrict__ active, const double *__restrict__ m,
+		const double *__restrict__ dtk, const double *__restrict__ ax,
+		const double *__restrict__ ay, const double *__restrict__ az)
+{
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= *T.ntop)
+		return;
+	const int lo = active_lower(active, nact, T.first[t]);
+	const int hi = active_lower(active, nact, T.first[t] + T.npart[t]);
+	if (lo >= hi)
+		return;
+	const double M = (double)T.mass[t];
+	float dx = T.dp[3 * t], dy = T.dp[3 * t + 1], dz = T.dp[3 * t + 2];
+	for (int g = lo; g < hi; g++) {
+		const int i = active[g];
+		const double mdt = m[i] * dtk[i];
+		dx = (float)((double)dx + mdt * ax[i] / M);
+		dy = (float)((double)dy + mdt * ay[i] / M);
+		dz = (float)((double)dz + mdt * az[i] / M);
+	}
+	T.dp[3 * t] = dx;
+	T.dp[3 * t + 1] = dy;
+	T.dp[3 * t + 2] = dz;
+	T.kicked[t] = 1;
+}
+
+/* Gravity_Tree_Update_Drift (tree_update.c:64-104): the updated nodes move
+ * by dt Dp and their Dp and flag are cleared; the kicked top nodes the
+ * same, in float */
+__global__ void k_tree_drift(size_t nn, int4 *__restrict__ nc,
+		double4 *__restrict__ na, double4 *__restrict__ dpn, double dt)
+{
+	const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= nn)
+		return;
+	const int4 c = nc[j];
+	if (!((c.z >> TDV_BIT_UPDATED) & 1))
+		return;
+	const double4 d = dpn[j];
+	double4 a = na[j];
+	a.x = a.x + dt * d.x;
+	a.y = a.y + dt * d.y;
+	a.z = a.z + dt * d.z;
+	na[j] = a;
+	dpn[j] = make_double4(0, 0, 0, 0);
+	nc[j].z = c.z & ~(1 << TDV_BIT_UPDATED);
+}
+
+__global__ void k_top_drift(const TopNodes T, double dt)
+{
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= *T.ntop || !T.kicked[t])
+		return;
+	for (int d = 0; d < 3; d++) {
+		T.com[3 * t + d] = (float)((double)T.com[3 * t + d]
+				+ dt * (double)T.dp[3 * t + d]);
+		T.dp[3 * t + d] = 0.f;
+	}
+	T.kicked[t] = 0;
 }
 
 /* while (x < 0) x += box; while (x >= box) x -= box (periodic.c:33-49),
@@ -368,8 +484,8 @@ int tdv_set_timeline(tdv_context *c, const tdv_timeline *tl)
 		natural_spline(x, yk, ck, TL_KNOTS);
 		natural_spline(x, yd, cd, TL_KNOTS);
 	}
-	if (!c->tl_spline)
-		TDV_HIP(c, hipMalloc((void **)&c->tl_spline, h.size() * 8));
+	if (!c->tl_spline)                   /* + 2 ints for the bin min/max */
+		TDV_HIP(c, hipMalloc((void **)&c->tl_spline, h.size() * 8 + 8));
 	TDV_HIP(c, hipMemcpyAsync(c->tl_spline, h.data(), h.size() * 8,
 			hipMemcpyHostToDevice, c->stream));
 	TDV_HIP(c, hipStreamSynchronize(c->stream));
@@ -407,17 +523,30 @@ int tdv_get_integer_times(tdv_context *c, uint32_t *it_drift,
 	return tdv_check(c, hipStreamSynchronize(c->stream), "get_integer_times");
 }
 
-int tdv_kick_halfstep(tdv_context *c)
+int tdv_kick_halfstep(tdv_context *c, int update_tree)
 {
 	if (!c || !c->n || !c->have_timeline)
 		return tdv_fail(c, TDV_ERR_ARG, "tdv_kick_halfstep: no particles or "
 				"no timeline (tdv_set_timeline)");
+	if (update_tree && !c->have_tree)
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_kick_halfstep: tree kicks "
+				"without a tree");
 	(void)hipEventRecord(c->ev[EV_KICK0], c->stream);
 	if (c->nactive)
 		hipLaunchKernelGGL(k_kick_bins, dim3(tdv_blocks((size_t)c->nactive,
 				256)), dim3(256), 0, c->stream, c->nactive, c->active,
 				c->time_bin, c->it_kick, c->acc[0], c->acc[1], c->acc[2],
-				c->vel[0], c->vel[1], c->vel[2], timeline_dev(c));
+				c->vel[0], c->vel[1], c->vel[2], c->kick_dt, timeline_dev(c));
+	if (update_tree && c->nactive) {
+		if (c->nnodes)
+			hipLaunchKernelGGL(k_tree_kicks, dim3(tdv_blocks(c->nnodes, 256)),
+					dim3(256), 0, c->stream, c->nnodes, c->node_c, c->node_a,
+					c->node_dp, c->nactive, c->active, c->mass, c->kick_dt,
+					c->acc[0], c->acc[1], c->acc[2]);
+		hipLaunchKernelGGL(k_top_kicks, dim3(tdv_blocks((size_t)c->ntop, 64)),
+				dim3(64), 0, c->stream, c->top, c->nactive, c->active, c->mass,
+				c->kick_dt, c->acc[0], c->acc[1], c->acc[2]);
+	}
 	(void)hipEventRecord(c->ev[EV_KICK1], c->stream);
 	TDV_HIP(c, hipGetLastError());
 	TDV_HIP(c, hipStreamSynchronize(c->stream));
@@ -469,7 +598,7 @@ int tdv_set_new_timebins(tdv_context *c, const tdv_timestep_params *p,
 	d.hubble = p->hubble_parameter;
 	d.accel_factor = p->grav_accel_factor;
 	d.max_active_bin = p->max_active_bin;
-	int *mm = reinterpret_cast<int *>(c->q_arrived);
+	int *mm = reinterpret_cast<int *>(c->tl_spline + 5 * TL_KNOTS);
 	hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(1), 0, c->stream, mm);
 	hipLaunchKernelGGL(k_timebins, dim3(tdv_blocks(c->n, 256)), dim3(256), 0,
 			c->stream, c->n, c->gacc[0], c->gacc[1], c->gacc[2], c->time_bin,
@@ -493,3 +622,18 @@ int tdv_set_new_timebins(tdv_context *c, const tdv_timestep_params *p,
 }
 
 } /* extern "C" */
+
+extern "C" int tdv_tree_update_drift(tdv_context *c, double dt)
+{
+	if (!c || !c->have_tree || !std::isfinite(dt))
+		return tdv_fail(c, TDV_ERR_ARG, "tdv_tree_update_drift: no tree or "
+				"dt not finite");
+	if (c->nnodes)
+		hipLaunchKernelGGL(k_tree_drift, dim3(tdv_blocks(c->nnodes, 256)),
+				dim3(256), 0, c->stream, c->nnodes, c->node_c, c->node_a,
+				c->node_dp, dt);
+	hipLaunchKernelGGL(k_top_drift, dim3(tdv_blocks((size_t)c->ntop, 64)),
+			dim3(64), 0, c->stream, c->top, dt);
+	TDV_HIP(c, hipGetLastError());
+	return tdv_check(c, hipStreamSynchronize(c->stream), "tree drift");
+}

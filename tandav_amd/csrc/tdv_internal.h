@@ -30,6 +30,7 @@
 #define TDV_NE3 (TDV_NE1 * TDV_NE1 * TDV_NE1)
 #define TDV_BIT_LOCAL 9
 #define TDV_BIT_TOP 10
+#define TDV_BIT_UPDATED 11
 #define TDV_MAX_TOP 65536       /* top-node records a context accepts */
 
 /* stage events of a context (stage_ms: keys, sort, reorder, top nodes,
@@ -49,6 +50,8 @@ struct TopNodes {            /* device arrays, top_cap entries each */
 	float *pos;              /* 3 per node */
 	float *mass;
 	float *com;              /* 3 per node */
+	float *dp;               /* 3 per node: TNode.Dp, the tree kicks */
+	int *kicked;             /* TNode.Level < 0: kicked since the drift */
 	int *ptop;               /* per particle: its top node */
 	int *ntop;               /* [1] */
 	int *nnodes;             /* [1] total tree nodes */
@@ -69,6 +72,7 @@ struct tdv_context {
 	int *tree_parent, *time_bin;
 	uint32_t *id;
 	uint32_t *it_drift, *it_kick;    /* P.It_Drift_Pos, P.It_Kick_Pos */
+	double *kick_dt;                 /* each particle's last kick factor */
 	double4 *pm;                     /* {x, y, z, m} for the walk */
 	int *active;                     /* active list */
 	int nactive;
@@ -77,13 +81,19 @@ struct tdv_context {
 	unsigned *d_queue;               /* work-queue counter of the walk */
 	double4 *partial;                /* split walk: TDV_SPLIT_ITEMS x 64 */
 	double *q_s6;                    /* nodes' FP64 second moments, 6 each */
-	unsigned *q_arrived;             /* level counts, fills, offsets: 192 */
+	unsigned *q_arrived;             /* per level: counts [0, 64), fill
+	                                    counters [64, 128), list offsets
+	                                    [128, 192) */
 	int *lvl_nodes;                  /* node indices grouped by level */
-	int lvl_off[64], lvl_cnt[64];    /* per level: first entry, count */
 	int lvl_ok;                      /* the lists match the tree */
 	hipGraph_t graph;                /* captured regular-step walk */
 	hipGraphExec_t graph_exec;
 	int capturing;
+	hipGraph_t step_graph;           /* captured whole step */
+	hipGraphExec_t step_exec;
+	int graph_mode;                  /* capturing the whole step: fixed
+	                                    capacities, counts on the device */
+	double step_kick, step_drift;    /* the captured step's factors */
 
 	/* sort scratch */
 	uint64_t *sk[2];
@@ -107,6 +117,7 @@ struct tdv_context {
 	double4 *node_a, *node_b;
 	int4 *node_c;                    /* DNext, Npart, Bitfield, first part. */
 	int *node_up;                    /* DUp */
+	double4 *node_dp;                /* Dp of the tree kicks (tree.h:31) */
 
 	TopNodes top;
 	int ntop, top_cap;
@@ -175,6 +186,7 @@ int tdv_walk64(tdv_context *c, int use_bh);
 int tdv_walk64_ewald(tdv_context *c, int use_bh);
 int tdv_walk32(tdv_context *c, int use_bh);
 void tdv_launch_safe_last_accel_dev(tdv_context *c);
+void tdv_launch_kick_drift(tdv_context *c, double kick, double drift);
 void tdv_graph_drop(tdv_context *c);
 #define TDV_QUEUE_GRID 8192              /* 256 CUs x 4 SIMDs x 8 waves;
                                             waves beyond those resident

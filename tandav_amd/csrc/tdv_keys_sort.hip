@@ -345,6 +345,8 @@ struct SortWork {
 	unsigned *d_err;             /* TDV_EBIT_* flags */
 	uint64_t *hi_tmp;            /* n words: the tie fallback's high words */
 	uint32_t *p_tmp;             /* n: the tie fallback's first order */
+	bool no_sync;                /* captured: no read back, no fallback (the
+	                                tie flag stays in d_err for the caller) */
 };
 
 static SortWork context_work(tdv_context *c, const uint32_t *perm_out)
@@ -362,6 +364,7 @@ static SortWork context_work(tdv_context *c, const uint32_t *perm_out)
 	w.hi_tmp = reinterpret_cast<uint64_t *>(c->tmp_f64);
 	w.p_tmp = c->perm == perm_out ? reinterpret_cast<uint32_t *>(c->lohi)
 			: c->perm;
+	w.no_sync = c->graph_mode != 0;
 	return w;
 }
 
@@ -466,6 +469,11 @@ static int sort_keys_work(tdv_context *c, const SortWork &w, size_t n,
 	TDV_TRY(radix_sort_u64(c, w, n, keys, 2, 1, nullptr, &res));
 	hipLaunchKernelGGL(k_fix_ties, dim3(tdv_blocks(n, 256)), dim3(256), 0,
 			w.stream, n, w.sk[res], w.sv[res], keys, w.d_err);
+	if (w.no_sync) {
+		TDV_HIP(c, hipMemcpyAsync(perm_out, w.sv[res], n * sizeof(uint32_t),
+				hipMemcpyDeviceToDevice, w.stream));
+		return tdv_check(c, hipGetLastError(), "sort");
+	}
 	unsigned e = 0;
 	TDV_HIP(c, hipMemcpyAsync(&e, w.d_err, sizeof(e), hipMemcpyDeviceToHost,
 			w.stream));
@@ -570,6 +578,7 @@ extern "C" int tdv_sort_keys_noalloc(size_t n, const uint64_t *keys,
 	w.hist_cap = NA_HIST_CAP;
 	w.scan_part = reinterpret_cast<uint32_t *>(base + NA_PART);
 	w.d_err = reinterpret_cast<unsigned *>(base + NA_ERR);
+	w.no_sync = false;
 	/* the previous call's flags stay in the array: clear them */
 	if (hipMemsetAsync(w.d_err, 0, sizeof(unsigned), w.stream) != hipSuccess) {
 		(void)hipGetLastError();

@@ -62,6 +62,20 @@ __global__ void k_drift(size_t n, double *__restrict__ x,
 	pm[i] = make_double4(px, py, pz, o.w);
 }
 
+/* the kernels of one global step: every particle active */
+void tdv_launch_kick_drift(tdv_context *c, double kick, double drift)
+{
+	if (kick != 0 && c->n)
+		hipLaunchKernelGGL(k_kick, dim3(tdv_blocks(c->n, 256)), dim3(256), 0,
+				c->stream, (int)c->n, c->active, c->acc[0], c->acc[1],
+				c->acc[2], c->vel[0], c->vel[1], c->vel[2], kick);
+	if (drift != 0 && c->n)
+		hipLaunchKernelGGL(k_drift, dim3(tdv_blocks(c->n, 256)), dim3(256), 0,
+				c->stream, c->n, c->pos[0], c->pos[1], c->pos[2], c->vel[0],
+				c->vel[1], c->vel[2], drift, c->par.periodic,
+				c->par.boxsize, c->pm);
+}
+
 extern "C" int tdv_kick_drift(tdv_context *c, double kick, double drift)
 {
 	if (!c || !c->n || !std::isfinite(kick) || !std::isfinite(drift))
@@ -72,15 +86,7 @@ extern "C" int tdv_kick_drift(tdv_context *c, double kick, double drift)
 				"needs every particle active; hierarchical steps use "
 				"tdv_kick_halfstep and tdv_drift_to_sync_point");
 	(void)hipEventRecord(c->ev[EV_KICK0], c->stream);
-	if (kick != 0 && c->nactive)
-		hipLaunchKernelGGL(k_kick, dim3(tdv_blocks((size_t)c->nactive, 256)),
-				dim3(256), 0, c->stream, c->nactive, c->active, c->acc[0],
-				c->acc[1], c->acc[2], c->vel[0], c->vel[1], c->vel[2], kick);
-	if (drift != 0)
-		hipLaunchKernelGGL(k_drift, dim3(tdv_blocks(c->n, 256)), dim3(256), 0,
-				c->stream, c->n, c->pos[0], c->pos[1], c->pos[2], c->vel[0],
-				c->vel[1], c->vel[2], drift, c->par.periodic,
-				c->par.boxsize, c->pm);
+	tdv_launch_kick_drift(c, kick, drift);
 	(void)hipEventRecord(c->ev[EV_KICK1], c->stream);
 	TDV_HIP(c, hipGetLastError());
 	TDV_HIP(c, hipStreamSynchronize(c->stream));

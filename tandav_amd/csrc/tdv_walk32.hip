@@ -77,13 +77,14 @@ __device__ __forceinline__ unsigned fixp(double x, double o, double S)
 }
 
 
-__global__ void k_prep32_nodes(size_t nn, const double4 *__restrict__ na,
+__global__ void k_prep32_nodes(size_t bound, const int *__restrict__ nn,
+		const double4 *__restrict__ na,
 		const double4 *__restrict__ nb, double o0, double o1, double o2,
 		double S, double mscale, int4 *__restrict__ a32,
 		int4 *__restrict__ b32)
 {
 	const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-	if (j >= nn)
+	if (j >= bound || (long long)j >= (long long)*nn)
 		return;
 	const double4 a = na[j], b = nb[j];
 	a32[j] = make_int4((int)fixp(a.x, o0, S), (int)fixp(a.y, o1, S),
@@ -92,11 +93,11 @@ __global__ void k_prep32_nodes(size_t nn, const double4 *__restrict__ na,
 			(int)fixp(b.z, o2, S), __float_as_int((float)(b.w * S)));
 }
 
-__global__ void k_prep32_top(int ntop, const TopNodes T, double o0, double o1,
+__global__ void k_prep32_top(const TopNodes T, double o0, double o1,
 		double o2, double S, double mscale, int4 *__restrict__ t32)
 {
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= ntop)
+	if (t >= *T.ntop)
 		return;
 	const double m = (double)T.mass[t] * mscale;
 	t32[t] = make_int4((int)fixp(T.com[3 * t], o0, S),
@@ -247,18 +248,24 @@ __device__ __forceinline__ void moments_add_strided(
  * child covers and every eighth child term, then the eight partial sums
  * meet in a fixed butterfly (the same order on every run; every lane ends
  * with the same sums, addition being commutative). */
-__global__ void __launch_bounds__(256) k_moments_level(int count,
+__global__ void __launch_bounds__(256) k_moments_level(int lv,
+		const unsigned *__restrict__ lcnt, const int *__restrict__ loff,
 		const int *__restrict__ list, const int4 *__restrict__ nc,
 		const double4 *__restrict__ na, const double4 *__restrict__ pm,
 		double *__restrict__ s6, double scale, float4 *__restrict__ q)
 {
-	const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-	const int t = gt >> 3, sub = gt & 7;
+	/* the level's nodes, count and list offset on the device; a fixed grid
+	 * strides over them */
+	const int count = (int)lcnt[lv];
+	const int base = loff[lv];
+	for (int t0 = (blockIdx.x * blockDim.x) >> 3; t0 < count;
+			t0 += (gridDim.x * blockDim.x) >> 3) {
+	const int t = t0 + (threadIdx.x >> 3), sub = threadIdx.x & 7;
 	const bool on = t < count;          /* the same for a node's 8 lanes */
 	double m6[6] = { 0, 0, 0, 0, 0, 0 };
 	int j = 0;
 	if (on) {
-		j = list[t];
+		j = list[base + t];
 		const int4 c = nc[j];
 		const double4 a = na[j];
 		if (c.x > 1) {
@@ -297,15 +304,33 @@ __global__ void __launch_bounds__(256) k_moments_level(int count,
 			s6[6 * (size_t)j + k] = m6[k];
 		moments_store(m6, scale, q + 2 * j);
 	}
+	}
+}
+
+/* exclusive offsets of the per-level counts, and the fill counters
+ * cleared (one lane) */
+__global__ void k_level_offsets(const unsigned *__restrict__ cnt,
+		int *__restrict__ off, unsigned *__restrict__ fill)
+{
+	if (threadIdx.x != 0)
+		return;
+	int run = 0;
+	for (int lv = 0; lv < 64; lv++) {
+		off[lv] = run;
+		run += (int)cnt[lv];
+		fill[lv] = 0;
+	}
 }
 
 /* per-level node lists (terminators left out); counts are gathered per
  * block in LDS first, so the device-wide counters take one atomic per level
  * and block */
-__global__ void __launch_bounds__(256) k_level_count(int nn,
-		const int4 *__restrict__ nc, unsigned *__restrict__ cnt)
+__global__ void __launch_bounds__(256) k_level_count(int bound,
+		const int *__restrict__ nnd, const int4 *__restrict__ nc,
+		unsigned *__restrict__ cnt)
 {
 	__shared__ unsigned h[64];
+	const int nn = min(bound, *nnd);
 	const int j = blockIdx.x * blockDim.x + threadIdx.x;
 	if (threadIdx.x < 64)
 		h[threadIdx.x] = 0;
@@ -320,11 +345,13 @@ __global__ void __launch_bounds__(256) k_level_count(int nn,
 		atomicAdd(&cnt[threadIdx.x], h[threadIdx.x]);
 }
 
-__global__ void __launch_bounds__(256) k_level_fill(int nn,
-		const int4 *__restrict__ nc, const int *__restrict__ off,
-		unsigned *__restrict__ fill, int *__restrict__ list)
+__global__ void __launch_bounds__(256) k_level_fill(int bound,
+		const int *__restrict__ nnd, const int4 *__restrict__ nc,
+		const int *__restrict__ off, unsigned *__restrict__ fill,
+		int *__restrict__ list)
 {
 	__shared__ unsigned h[64], base[64];
+	const int nn = min(bound, *nnd);
 	const int j = blockIdx.x * blockDim.x + threadIdx.x;
 	if (threadIdx.x < 64)
 		h[threadIdx.x] = 0;
@@ -350,12 +377,12 @@ __global__ void __launch_bounds__(256) k_level_fill(int nn,
 /* top nodes: from the subtree's root, shifted to the float centre of mass
  * (S about a point P = S_cm + M (c - P)(c - P)^T), or from the particles of
  * a top node without subtree */
-__global__ void k_moments_top(int ntop, const TopNodes T, int vs,
+__global__ void k_moments_top(const TopNodes T, int vs,
 		const double4 *__restrict__ na, const double4 *__restrict__ pm,
 		const double *__restrict__ s6, double scale, float4 *__restrict__ q)
 {
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= ntop)
+	if (t >= *T.ntop || *T.nnodes < 0)
 		return;
 	const double cx = T.com[3 * t], cy = T.com[3 * t + 1],
 			cz = T.com[3 * t + 2];
@@ -671,6 +698,9 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 		const int4 *__restrict__ top32, const float4 *__restrict__ q32,
 		const float4 *__restrict__ tq32, const float4 *__restrict__ p32o)
 {
+	if (*T.nnodes < 0)                   /* the build overflowed: no walk */
+		return;
+	const int ntop = *T.ntop;
 	const int gid = group * 64 + (int)threadIdx.x;
 	const bool live = gid < nact;
 	const int ip = live ? active[gid] : active[nact - 1];
@@ -681,7 +711,7 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 	Acc32 a = { 0.f, 0.f, 0.f, 0.0, 0.0, 0.0, 0, 0ull, 0ull };
 	const int every = w.carry_every;
 
-	for (int j = 0; j < w.ntop; j++) {
+	for (int j = 0; j < ntop; j++) {
 		const int4 tc = top32[j];
 		const float tm = __int_as_float(tc.w);
 		const float ts = w.dom_f / (float)(1 << T.level[j]);
@@ -1167,54 +1197,53 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	}
 	/* quadrupoles: masses G S^2, lengths S -> scale G S^4 */
 	const double qscale = mscale * S * S;
+	/* nodes the kernels may touch: the count of the last eager build, or
+	 * the arrays' capacity in a captured step (the count is read on the
+	 * device) */
+	const size_t bound = c->graph_mode ? c->node_cap : nn;
+	const int tgrid = c->graph_mode ? c->top_cap : c->ntop;
 	if (quad) {
-		if (nn && !c->lvl_ok) {
-			/* the tree's per-level node lists, once per build (a host sync;
-			 * a captured walk requires them made) */
-			if (c->capturing)
-				return tdv_fail(c, TDV_ERR_ARG, "capture: levels missing");
+		if (bound && !c->lvl_ok) {
+			/* the tree's per-level node lists, once per build, on the
+			 * device: counts, offsets, fill */
 			unsigned *cnt = c->q_arrived, *fill = c->q_arrived + 64;
 			int *off = reinterpret_cast<int *>(c->q_arrived + 128);
 			hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(128), 0, c->stream,
 					128, c->q_arrived);
-			hipLaunchKernelGGL(k_level_count, dim3(tdv_blocks(nn, 256)),
-					dim3(256), 0, c->stream, (int)nn, c->node_c, cnt);
-			unsigned h[64];
-			TDV_HIP(c, hipMemcpyAsync(h, cnt, sizeof(h),
-					hipMemcpyDeviceToHost, c->stream));
-			TDV_HIP(c, hipStreamSynchronize(c->stream));
-			int run = 0;
-			for (int lv = 0; lv < 64; lv++) {
-				c->lvl_off[lv] = run;
-				c->lvl_cnt[lv] = (int)h[lv];
-				run += (int)h[lv];
-			}
-			TDV_HIP(c, hipMemcpyAsync(off, c->lvl_off, 64 * sizeof(int),
-					hipMemcpyHostToDevice, c->stream));
-			hipLaunchKernelGGL(k_level_fill, dim3(tdv_blocks(nn, 256)),
-					dim3(256), 0, c->stream, (int)nn, c->node_c, off, fill,
-					c->lvl_nodes);
-			TDV_HIP(c, hipStreamSynchronize(c->stream));   /* c->lvl_off */
+			hipLaunchKernelGGL(k_level_count, dim3(tdv_blocks(bound, 256)),
+					dim3(256), 0, c->stream, (int)bound, c->top.nnodes,
+					c->node_c, cnt);
+			hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(64), 0,
+					c->stream, cnt, off, fill);
+			hipLaunchKernelGGL(k_level_fill, dim3(tdv_blocks(bound, 256)),
+					dim3(256), 0, c->stream, (int)bound, c->top.nnodes,
+					c->node_c, off, fill, c->lvl_nodes);
 			c->lvl_ok = 1;
 		}
-		for (int lv = 63; nn && lv >= 0; lv--)
-			if (c->lvl_cnt[lv] > 0)
-				hipLaunchKernelGGL(k_moments_level,
-						dim3(tdv_blocks(8 * (size_t)c->lvl_cnt[lv], 256)),
-						dim3(256), 0, c->stream, c->lvl_cnt[lv],
-						c->lvl_nodes + c->lvl_off[lv], c->node_c, c->node_a,
-						c->pm, c->q_s6, qscale, c->q32);
-		hipLaunchKernelGGL(k_moments_top, dim3(tdv_blocks((size_t)c->ntop,
-				64)), dim3(64), 0, c->stream, c->ntop, c->top,
-				c->par.vector_size, c->node_a, c->pm, c->q_s6, qscale, c->tq32);
+		/* bottom-up: a level's nodes take their children's moments */
+		if (bound) {
+			const unsigned grid = (unsigned)std::min(
+					(size_t)tdv_blocks(8 * bound, 256), (size_t)2048);
+			for (int lv = TDV_NT; lv >= 0; lv--)
+				hipLaunchKernelGGL(k_moments_level, dim3(grid), dim3(256), 0,
+						c->stream, lv, c->q_arrived,
+						reinterpret_cast<const int *>(c->q_arrived + 128),
+						c->lvl_nodes, c->node_c, c->node_a, c->pm, c->q_s6,
+						qscale, c->q32);
+		}
+		if (tgrid)
+			hipLaunchKernelGGL(k_moments_top, dim3(tdv_blocks((size_t)tgrid,
+					64)), dim3(64), 0, c->stream, c->top, c->par.vector_size,
+					c->node_a, c->pm, c->q_s6, qscale, c->tq32);
 	}
-	hipLaunchKernelGGL(k_prep32_top, dim3(tdv_blocks((size_t)c->ntop, 64)),
-			dim3(64), 0, c->stream, c->ntop, c->top, o[0], o[1], o[2], S,
-			mscale, c->top32);
-	if (nn)
-		hipLaunchKernelGGL(k_prep32_nodes, dim3(tdv_blocks(nn, 256)),
-				dim3(256), 0, c->stream, nn, c->node_a, c->node_b, o[0], o[1],
-				o[2], S, mscale, c->a32, c->b32);
+	if (tgrid)
+		hipLaunchKernelGGL(k_prep32_top, dim3(tdv_blocks((size_t)tgrid, 64)),
+				dim3(64), 0, c->stream, c->top, o[0], o[1], o[2], S, mscale,
+				c->top32);
+	if (bound)
+		hipLaunchKernelGGL(k_prep32_nodes, dim3(tdv_blocks(bound, 256)),
+				dim3(256), 0, c->stream, bound, c->top.nnodes, c->node_a,
+				c->node_b, o[0], o[1], o[2], S, mscale, c->a32, c->b32);
 	hipLaunchKernelGGL(k_prep32_particles, dim3(tdv_blocks(n, 256)), dim3(256),
 			0, c->stream, n, c->pm, o[0], o[1], o[2], S, mscale,
 			c->tree_parent, c->a32, c->p32, c->p32o);

@@ -161,12 +161,15 @@ def load_library(path=LIB_PATH):
     L.tdv_set_fp32_options.argtypes = [_ctx, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int]
     L.tdv_capture_walk.argtypes = [_ctx]
+    L.tdv_capture_step.argtypes = [_ctx, ctypes.c_double, ctypes.c_double]
+    L.tdv_launch_step.argtypes = [_ctx]
     L.tdv_set_timeline.argtypes = [_ctx, ctypes.POINTER(TdvTimeline)]
     L.tdv_comoving_tables.argtypes = [ctypes.POINTER(TdvTimeline), _dp, _dp,
                                       _dp]
     L.tdv_set_integer_times.argtypes = [_ctx, _vp, _vp]
     L.tdv_get_integer_times.argtypes = [_ctx, _vp, _vp]
-    L.tdv_kick_halfstep.argtypes = [_ctx]
+    L.tdv_kick_halfstep.argtypes = [_ctx, ctypes.c_int]
+    L.tdv_tree_update_drift.argtypes = [_ctx, ctypes.c_double]
     L.tdv_drift_to_sync_point.argtypes = [_ctx, ctypes.c_uint32]
     L.tdv_set_new_timebins.argtypes = [_ctx,
                                        ctypes.POINTER(TdvTimestepParams),
@@ -435,6 +438,16 @@ class Engine:
         """Replay the captured walk."""
         self._ok(self.lib.tdv_launch_walk(self._c), "tdv_launch_walk")
 
+    def capture_step(self, kick=0.0, drift=0.0):
+        """Capture the whole regular step (decomposition, build, FP32 walk,
+        kick-drift with these factors) as one HIP graph."""
+        self._ok(self.lib.tdv_capture_step(self._c, float(kick),
+                                           float(drift)), "tdv_capture_step")
+
+    def launch_step(self):
+        """Replay the captured step."""
+        self._ok(self.lib.tdv_launch_step(self._c), "tdv_launch_step")
+
     def forcetest_exact(self, idx, pairs=False):
         """forcetest with the exact periodic force (the Ewald lattice sum
         the table samples) instead of the table's interpolation; pairs=True
@@ -662,9 +675,17 @@ class Engine:
             "tdv_get_integer_times")
         return a, b
 
-    def kick_halfstep(self):
-        """Kick_First/Second_Halfstep (kick.c:9-70) on the device."""
-        self._ok(self.lib.tdv_kick_halfstep(self._c), "tdv_kick_halfstep")
+    def kick_halfstep(self, update_tree=False):
+        """Kick_First/Second_Halfstep (kick.c:9-70) on the device; with
+        update_tree the tree takes the kicks (Gravity_Tree_Update_Kicks,
+        tree_update.c:12-57)."""
+        self._ok(self.lib.tdv_kick_halfstep(self._c, int(bool(update_tree))),
+                 "tdv_kick_halfstep")
+
+    def tree_update_drift(self, dt):
+        """Gravity_Tree_Update_Drift (tree_update.c:64-104) on the device."""
+        self._ok(self.lib.tdv_tree_update_drift(self._c, float(dt)),
+                 "tdv_tree_update_drift")
 
     def drift_to_sync_point(self, it_next):
         """Drift_To_Sync_Point (drift.c:12-64) to Int_Time.Next = it_next."""

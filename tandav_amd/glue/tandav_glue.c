@@ -100,6 +100,23 @@ static tdv_context *context(void)
 			: TDV_WALK_FP64;
 	p.ewald_size = 0.125;
 	p.ewald_accuracy = 0.1;
+	if (p.precision == TDV_WALK_FP32) {
+		/* the FP32 group walk's own settings, the ones its accuracy is
+		 * stated for (p99 |da|/|a| <= 2e-3 against the exact periodic force
+		 * on the 128^3 and 256^3 boxes): the next-order relative criterion
+		 * M s^3 > r^5 |a| alpha / G at alpha 0.01, quadrupoles, the Ewald
+		 * correction at centres of mass to second order (ewald_accuracy 1);
+		 * TANDAV_FP32_ALPHA, TANDAV_FP32_CRITERION (2 or 3) and
+		 * TANDAV_FP32_EWALD_ACCURACY override them */
+		const char *a = getenv("TANDAV_FP32_ALPHA");
+		const char *c = getenv("TANDAV_FP32_CRITERION");
+		const char *e = getenv("TANDAV_FP32_EWALD_ACCURACY");
+		p.alpha_rel = a ? atof(a) : 0.01;
+		p.fp32_criterion = c ? atoi(c) : 3;
+		p.ewald_accuracy = e ? atof(e) : 1.0;
+		p.fp32_order = 2;
+		p.compat = 0;
+	}
 	const int rc = tdv_create(&p, &Ctx);
 	Assert(rc == TDV_OK, "tandav engine: tdv_create failed (%d) on device %d",
 			rc, p.device);

@@ -10,7 +10,7 @@
   p99 |da|/|a| <= 2e-3 against FP64 direct summation with the exact
   periodic force on 512 sampled particles.
 * Clustered 2^23 box (256 Hernquist halos + 20% background), FP32 walk at
-  alpha 0.003: eager walks and replays of the captured walk graph at 100%,
+  alpha 0.002: eager walks and replays of the captured walk graph at 100%,
   3.8% and 0.05% of the particles active (time bins by |a| rank), the
   replay bit-identical to the eager walk and p99 <= 2e-3 against the exact
   force on 256 sampled active particles per set.
@@ -129,12 +129,18 @@ def _time_bins(acc, n):
     return bins
 
 
+# the halo cores, which make up the sparse active sets, need a tighter
+# criterion than the whole box: at 0.003 the 3.8% set (bins <= 4) had p99
+# 2.4e-3 while all particles had 1.2e-3
+C4_ALPHA = 0.002
+
+
 def test_c4_clustered_substeps_eager_and_graph_at_full_size():
     n = 1 << 23
     pos, _, m = ics.clustered_box(n, nhalo=256, seed=11)
     soft = 1.0 / (40.0 * n ** (1.0 / 3.0))
     say("c4: %d particles, prepare + regular step" % n)
-    e = _fp32(softening=soft, alpha_rel=0.003, fp32_queue=1)
+    e = _fp32(softening=soft, alpha_rel=C4_ALPHA, fp32_queue=1)
     e.set_particles(pos, m)
     e.setup_domain()
     e.domain_decomposition()

@@ -208,3 +208,25 @@ def test_binding_momentum_check_output(tmp_path, capfd):
             if l.startswith("Total change in momentum due to gravity:")]
     assert vals == pytest.approx([1.0, 1 / 3, 1 / 5], rel=1e-5), out
     assert out.count("Tree acceleration  done") == 3, out
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_binding_fp32_walk_accuracy(periodic):
+    """TANDAV_WALK=fp32 in a fresh program (the binding reads it when it
+    creates its context): Tandav's calling sequence through the binding
+    with the FP32 group walk at the binding's settings holds p99
+    |da|/|a| <= 2e-3 against FP64 direct summation (exact periodic force)."""
+    import json
+    import subprocess
+    env = dict(os.environ, TANDAV_WALK="fp32")
+    for k in ("TANDAV_FP32_ALPHA", "TANDAV_FP32_CRITERION",
+              "TANDAV_FP32_EWALD_ACCURACY"):
+        env.pop(k, None)
+    script = os.path.join(os.path.dirname(__file__), "glue", "fp32_step.py")
+    r = subprocess.run([sys.executable, script, "1" if periodic else "0"],
+                       env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    print(res)
+    assert res["p99"] <= 2e-3
+    assert res["cost_min"] > 0

@@ -132,3 +132,46 @@ def test_device_active_list_matches_host_list():
         np.testing.assert_array_equal(ga["cost"], gb["cost"])
     with pytest.raises(TDVError):
         b.make_active_list(0)
+
+
+def test_step_graph_matches_eager_steps():
+    """The whole regular step captured as one HIP graph (decomposition,
+    build, FP32 walk, kick-drift of one global step) and replayed three
+    times, with eager work between the replays, leaves every column, the
+    keys, the top nodes and the tree bit for bit as eager steps do."""
+    pos, vel, m = ics.zeldovich_box(48, seed=7)
+    engines = []
+    for _ in range(2):
+        e = Engine(periodic=True, softening=1 / 1920., alpha_rel=0.01,
+                   precision="fp32", compat=False, ewald_size=0.125,
+                   ewald_accuracy=1.0, fp32_order=2, fp32_criterion=3)
+        e.set_particles(pos, m, vel=vel * 0.3)
+        e.setup_domain()
+        e.domain_decomposition()
+        e.compute_acceleration(True)
+        e.domain_decomposition()               # one eager step
+        e.compute_acceleration(False)
+        e.kick_drift(0.002, 0.004)
+        engines.append(e)
+    eager, graph = engines
+    graph.capture_step(0.002, 0.004)
+    cols = ("pos", "vel", "acc", "grav_acc", "cost", "last_acc_mag",
+            "tree_parent", "key", "id")
+    for step in range(3):
+        eager.domain_decomposition()
+        eager.compute_acceleration(False)
+        eager.kick_drift(0.002, 0.004)
+        graph.launch_step()
+        a, b = eager.get(*cols), graph.get(*cols)
+        for f in cols:
+            np.testing.assert_array_equal(a[f], b[f],
+                                          err_msg="%s step %d" % (f, step))
+        assert eager.interaction_counts() == graph.interaction_counts()
+        ta, tb = eager.tree(), graph.tree()
+        assert len(ta) == len(tb)
+        for f in ta.dtype.names:
+            np.testing.assert_array_equal(ta[f], tb[f], err_msg=f)
+        np.testing.assert_array_equal(eager.top_nodes(), graph.top_nodes())
+        # eager work between replays
+        graph.forcetest([0, 5])
+        graph.get("pos")
