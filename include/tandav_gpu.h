@@ -83,6 +83,13 @@ typedef struct {
 	                          group's subtree walks are cut into, one wave per
 	                          part (summed in part order); 0 = up to 64 parts,
 	                          1 = never split, n >= 2 = up to n (<= 256)      */
+	int fp32_bundle_min;   /* FP32 walk: a particle bundle (collapsed leaf)
+	                          of at least this many particles is tested with
+	                          the opening criterion like a node and taken
+	                          whole (monopole + quadrupole at its centre of
+	                          mass) when no lane opens it; 0 = 4, -1 = never
+	                          (every bundle particle by particle, as the
+	                          reference's walk, Gravity/tree_accel.c:219-240) */
 } tdv_params;
 
 enum { TDV_WALK_FP64 = 0, TDV_WALK_FP32 = 1 };

@@ -316,6 +316,8 @@ int tdv_create(const tdv_params *p, tdv_context **out)
 		return TDV_ERR_ARG;
 	if (p->fp32_split < 0 || p->fp32_split > 256)
 		return TDV_ERR_ARG;
+	if (p->fp32_bundle_min < -1 || p->fp32_bundle_min > 64)
+		return TDV_ERR_ARG;
 	tdv_context *c = new (std::nothrow) tdv_context();
 	if (!c)
 		return TDV_ERR_NOMEM;

@@ -41,7 +41,8 @@ class TdvParams(ctypes.Structure):
                 ("fp32_carry", ctypes.c_int),
                 ("fp32_waves", ctypes.c_int),
                 ("fp32_queue", ctypes.c_int),
-                ("fp32_split", ctypes.c_int)]
+                ("fp32_split", ctypes.c_int),
+                ("fp32_bundle_min", ctypes.c_int)]
 
 
 class TdvTimeline(ctypes.Structure):
@@ -225,7 +226,7 @@ class Engine:
                  compat=True, device=0, ewald_table=None, precision="fp64",
                  ewald_size=0.125, ewald_accuracy=0.1, fp32_order=2,
                  fp32_criterion=2, fp32_carry=16, fp32_waves=0,
-                 fp32_queue=0, fp32_split=0):
+                 fp32_queue=0, fp32_split=0, fp32_bundle_min=0):
         self.lib = load_library()
         prec = {"fp64": 0, "fp32": 1}[precision]
         self.params = TdvParams(int(periodic), float(boxsize), float(gravity),
@@ -235,7 +236,8 @@ class Engine:
                                 float(ewald_size), float(ewald_accuracy),
                                 int(fp32_order), int(fp32_criterion),
                                 int(fp32_carry), int(fp32_waves),
-                                int(fp32_queue), int(fp32_split))
+                                int(fp32_queue), int(fp32_split),
+                                int(fp32_bundle_min))
         self._c = _ctx()
         rc = self.lib.tdv_create(ctypes.byref(self.params),
                                  ctypes.byref(self._c))

@@ -236,3 +236,39 @@ def test_fp32_walk_against_exact_ewald():
     r32, r64 = rel_err(out[1][idx], ref), rel_err(out[0][idx], ref)
     assert np.percentile(r32, 99) <= 2e-3
     assert np.percentile(r32, 99) < 0.5 * np.percentile(r64, 99)
+
+
+def test_fp32_bundles_taken_whole():
+    """Bundles (collapsed leaves) that no lane's criterion opens are taken
+    whole at their centre of mass (fp32_bundle_min, default 4): far fewer
+    interactions than taking every bundle particle by particle (-1, the
+    reference's walk), with the error against the exact periodic force
+    still within the bar."""
+    pos, _, m = ics.zeldovich_box(48, seed=7)
+    soft = 1 / 1920.
+    out = {}
+    for bmin in (-1, 0, 64):
+        e = Engine(periodic=True, softening=soft, alpha_rel=0.01,
+                   precision="fp32", compat=False, ewald_size=0.125,
+                   ewald_accuracy=1.0, fp32_order=2, fp32_criterion=3,
+                   fp32_bundle_min=bmin)
+        e.set_particles(pos, m)
+        e.setup_domain()
+        e.domain_decomposition()
+        e.compute_acceleration(True)
+        e.domain_decomposition()
+        e.compute_acceleration(False)
+        out[bmin] = (e.get("acc")["acc"], e.interaction_counts()[0])
+        if bmin == -1:
+            g = np.random.Generator(np.random.Philox(5))
+            idx = np.sort(g.choice(len(m), 256, replace=False))
+            ref = e.forcetest_exact(idx)
+        e.close()
+    assert out[0][1] < 0.5 * out[-1][1]
+    assert out[0][1] <= out[64][1] <= out[-1][1]
+    for bmin in (-1, 0, 64):
+        err = rel_err(out[bmin][0][idx], ref)
+        assert np.percentile(err, 99) <= 2e-3, bmin
+    from tandav_amd import TDVError
+    with pytest.raises(TDVError):
+        Engine(precision="fp32", compat=False, fp32_bundle_min=65)
