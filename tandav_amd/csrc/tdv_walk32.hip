@@ -84,6 +84,9 @@ __device__ __forceinline__ unsigned fixp(double x, double o, double S)
 }
 
 
+/* The walk's node records: a32 = {CoM (fixed), m'}; b32 = {centre - CoM
+ * (fixed units, FP32 bits), size}, so a lane takes the offset to the
+ * centre with one add per axis. */
 __global__ void k_prep32_nodes(size_t bound, const int *__restrict__ nn,
 		const double4 *__restrict__ na,
 		const double4 *__restrict__ nb, double o0, double o1, double o2,
@@ -94,10 +97,15 @@ __global__ void k_prep32_nodes(size_t bound, const int *__restrict__ nn,
 	if (j >= bound || (long long)j >= (long long)*nn)
 		return;
 	const double4 a = na[j], b = nb[j];
-	a32[j] = make_int4((int)fixp(a.x, o0, S), (int)fixp(a.y, o1, S),
+	const int4 A = make_int4((int)fixp(a.x, o0, S), (int)fixp(a.y, o1, S),
 			(int)fixp(a.z, o2, S), __float_as_int((float)(a.w * mscale)));
-	b32[j] = make_int4((int)fixp(b.x, o0, S), (int)fixp(b.y, o1, S),
-			(int)fixp(b.z, o2, S), __float_as_int((float)(b.w * S)));
+	a32[j] = A;
+	const float s = (float)(b.w * S);
+	b32[j] = make_int4(
+			__float_as_int((float)(int)(fixp(b.x, o0, S) - (unsigned)A.x)),
+			__float_as_int((float)(int)(fixp(b.y, o1, S) - (unsigned)A.y)),
+			__float_as_int((float)(int)(fixp(b.z, o2, S) - (unsigned)A.z)),
+			__float_as_int(s));
 }
 
 __global__ void k_prep32_top(const TopNodes T, double o0, double o1,
@@ -681,6 +689,53 @@ __device__ __forceinline__ bool node_opens(float m, float s, float r2,
 			&& fabsf(sy) < 0.6f * s && fabsf(sz) < 0.6f * s);
 }
 
+/* node_opens given the node's inverse distance ri (the same decisions) */
+__device__ __forceinline__ bool node_opens_ri(float m, float s, float r2,
+		float ri, float sx, float sy, float sz, float crit,
+		const Walk32Params &w)
+{
+	const float es = w.eps_abs + s;
+	if (r2 < es * es)
+		return true;
+	if (w.use_bh)
+		return s * s > r2 * w.theta_bh;
+	const float ms2 = m * s * s;
+	const float lhs = w.crit3 ? ms2 * (s * ri) : ms2;
+	const float s6 = 0.6f * s;
+	return lhs > (r2 * crit) * r2 || (fabsf(sx) < s6 && fabsf(sy) < s6
+			&& fabsf(sz) < s6);
+}
+
+/* An accepted node (r >= |eps| + s, outside the softening length): its
+ * monopole and, with quadrupoles, T's term in one pass,
+ * a = r^-2 (m' + (5/2) (u.T u) r^-2) u - r^-4 T u, u = d / r. */
+__device__ __forceinline__ void node32(Acc32 &a, float m, float4 qa,
+		float qyz, float dx, float dy, float dz, float ri,
+		const Walk32Params &w)
+{
+	a.cost++;
+	const float ri2 = ri * ri;
+	if (!w.quad) {
+		const float fac = m * ri * ri2;
+		a.x = fmaf(fac, dx, a.x);
+		a.y = fmaf(fac, dy, a.y);
+		a.z = fmaf(fac, dz, a.z);
+		return;
+	}
+	const float ux = dx * ri, uy = dy * ri, uz = dz * ri;
+	const float txx = qa.x, tyy = qa.y, tzz = -(qa.x + qa.y), txy = qa.z,
+			txz = qa.w, tyz = qyz;
+	const float tux = fmaf(txx, ux, fmaf(txy, uy, txz * uz));
+	const float tuy = fmaf(txy, ux, fmaf(tyy, uy, tyz * uz));
+	const float tuz = fmaf(txz, ux, fmaf(tyz, uy, tzz * uz));
+	const float c = 2.5f * fmaf(ux, tux, fmaf(uy, tuy, uz * tuz));
+	const float ri4 = ri2 * ri2;
+	const float g = fmaf(c, ri4, m * ri2);
+	a.x = fmaf(-ri4, tux, fmaf(g, ux, a.x));
+	a.y = fmaf(-ri4, tuy, fmaf(g, uy, a.y));
+	a.z = fmaf(-ri4, tuz, fmaf(g, uz, a.z));
+}
+
 /* One wave walks `group` (64 active particles, one per lane) over part
  * `part` of `nparts`: every subtree the group walks is cut into nparts
  * contiguous node ranges and this wave takes range `part` of each, so
@@ -795,9 +850,11 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 			const float rx = fdiff(A.x, me.x), ry = fdiff(A.y, me.y),
 					rz = fdiff(A.z, me.z);
 			const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
-			const float sx = fdiff(B.x, me.x), sy = fdiff(B.y, me.y),
-					sz = fdiff(B.z, me.z);
-			if (!__any(node_opens(m, s, r2, sx, sy, sz, crit, w))) {
+			const float ri = __frsqrt_rn(r2);
+			const float sx = rx + __int_as_float(B.x),
+					sy = ry + __int_as_float(B.y),
+					sz = rz + __int_as_float(B.z);
+			if (!__any(node_opens_ri(m, s, r2, ri, sx, sy, sz, crit, w))) {
 				node = skip;             /* taken whole by an earlier part */
 				break;
 			}
@@ -826,17 +883,22 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 					const float rx = fdiff(A.x, me.x), ry = fdiff(A.y, me.y),
 							rz = fdiff(A.z, me.z);
 					const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
-					if (!__any(node_opens(m, s, r2, fdiff(B.x, me.x),
-							fdiff(B.y, me.y), fdiff(B.z, me.z), crit, w))) {
-						newton32(a, m, rx, ry, rz, w);
+					const float ri = __frsqrt_rn(r2);
+					if (!__any(node_opens_ri(m, s, r2, ri,
+							rx + __int_as_float(B.x), ry + __int_as_float(B.y),
+							rz + __int_as_float(B.z), crit, w))) {
+						float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f);
+						float qyz = 0.f;
 						if (w.quad) {
-							const float4 q0 = q32[2 * node];
-							const float qyz = q32[2 * node + 1].x;
-							quad32(a, q0, qyz, rx, ry, rz, w);
-							if (pending)
+							q0 = q32[2 * node];
+							qyz = q32[2 * node + 1].x;
+						}
+						node32(a, m, q0, qyz, rx, ry, rz, ri, w);
+						if (pending) {
+							if (w.quad)
 								ewald16q(a, tab, m, q0, qyz, rx, ry, rz, w);
-						} else if (pending) {
-							ewald16(a, tab, m, rx, ry, rz, w);
+							else
+								ewald16(a, tab, m, rx, ry, rz, w);
 						}
 						tick(a, every);
 						node++;
@@ -856,8 +918,9 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 							m = __int_as_float(A.w);
 					const float rx = fdiff(A.x, me.x), ry = fdiff(A.y, me.y),
 							rz = fdiff(A.z, me.z);
-					const bool ok = ewald_ok(m, s, rx, ry, rz, fdiff(B.x, me.x),
-							fdiff(B.y, me.y), fdiff(B.z, me.z), crit, true, w);
+					const bool ok = ewald_ok(m, s, rx, ry, rz,
+							rx + __int_as_float(B.x), ry + __int_as_float(B.y),
+							rz + __int_as_float(B.z), crit, true, w);
 					com_frame = __all(ok);
 					if (com_frame) {
 						if (w.quad)
@@ -931,18 +994,23 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 			const float rx = fdiff(A.x, me.x), ry = fdiff(A.y, me.y),
 					rz = fdiff(A.z, me.z);
 			const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
-			const float sx = fdiff(B.x, me.x), sy = fdiff(B.y, me.y),
-					sz = fdiff(B.z, me.z);
-			if (!__any(node_opens(m, s, r2, sx, sy, sz, crit, w))) {
-				newton32(a, m, rx, ry, rz, w);
+			const float ri = __frsqrt_rn(r2);
+			const float sx = rx + __int_as_float(B.x),
+					sy = ry + __int_as_float(B.y),
+					sz = rz + __int_as_float(B.z);
+			if (!__any(node_opens_ri(m, s, r2, ri, sx, sy, sz, crit, w))) {
+				float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f);
+				float qyz = 0.f;
 				if (w.quad) {
-					const float4 q0 = q32[2 * node];
-					const float qyz = q32[2 * node + 1].x;
-					quad32(a, q0, qyz, rx, ry, rz, w);
-					if (pending)
+					q0 = q32[2 * node];
+					qyz = q32[2 * node + 1].x;
+				}
+				node32(a, m, q0, qyz, rx, ry, rz, ri, w);
+				if (pending) {
+					if (w.quad)
 						ewald16q(a, tab, m, q0, qyz, rx, ry, rz, w);
-				} else if (pending) {
-					ewald16(a, tab, m, rx, ry, rz, w);
+					else
+						ewald16(a, tab, m, rx, ry, rz, w);
 				}
 				tick(a, every);
 				node += c.x;
@@ -1151,6 +1219,15 @@ __global__ void k_walk32_queue_reset(unsigned *__restrict__ queue,
 		counter[threadIdx.x] = 0;
 }
 
+/* the softening length in fixed units: negative (the reference's K1
+ * length), and at least half a unit, so the particle itself (r = 0)
+ * always takes the finite inner kernel */
+static double walk32_eps(const tdv_context *c, double S)
+{
+	const double ef = c->Epsilon * S;
+	return ef * ef < 0.5 ? -sqrt(0.5) : ef;
+}
+
 /* safe_last_accel (accel.c:28-43) with the active count on the device */
 __global__ void k_safe_last_accel_dev(const int *__restrict__ nact_dev,
 		const int *__restrict__ active, double *__restrict__ ax,
@@ -1305,12 +1382,7 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	w.use_bh = use_bh;
 	w.theta_bh = (float)c->par.theta_bh;
 	w.alpha_rel = (float)c->par.alpha_rel;
-	/* negative (the reference's K1 length), fixed units; at least half a
-	 * unit, so the particle itself (r = 0) always takes the finite inner
-	 * kernel */
-	double ef = c->Epsilon * S;
-	if (ef * ef < 0.5)
-		ef = -sqrt(0.5);
+	const double ef = walk32_eps(c, S);
 	w.eps2 = (float)(ef * ef);
 	w.eps_abs = (float)fabs(ef);
 	w.inv_eps2 = (float)(1.0 / (ef * ef));
