@@ -379,7 +379,7 @@ void tdv_destroy(tdv_context *c)
 	dfree(c->p32);
 	dfree(c->p32o);
 	dfree(c->a32);
-	dfree(c->b32);
+	dfree(c->rec32);
 	dfree(c->q32);
 	dfree(c->q_s6);
 	dfree(c->q_arrived);

@@ -128,7 +128,8 @@ struct tdv_context {
 	int have_ewald;
 
 	/* FP32 group walk: fixed-point records */
-	int4 *p32, *a32, *b32;
+	int4 *p32, *a32;
+	int4 *rec32;                     /* 64-byte node records, 4 int4 each */
 	float4 *p32o;                    /* offset from the leaf's CoM, mass' */
 	size_t n32_cap;
 	float4 *q32;                     /* 2 per node: traceless quadrupole */

@@ -84,14 +84,26 @@ __device__ __forceinline__ unsigned fixp(double x, double o, double S)
 }
 
 
-/* The walk's node records: a32 = {CoM (fixed), m'}; b32 = {centre - CoM
- * (fixed units, FP32 bits), size}, so a lane takes the offset to the
- * centre with one add per axis. */
+/* One node's (or bundle's) walk record, 64 bytes, fetched with one
+ * scalar load:
+ *   a = {CoM (fixed units), m'}
+ *   b = {centre - CoM (fixed units, FP32 bits), size (fixed units)}
+ *   c = {DNext (or -(first particle) - 1 for a bundle), Npart, Txx, Tyy}
+ *   q = {Txy, Txz, Tyz, 0}
+ * (T the scaled traceless quadrupole, zero when the walk takes monopoles
+ * only); a lane takes the offset to the centre with one add per axis. */
+struct __align__(64) Node32 {
+	int4 a, b, c, q;
+};
+
+/* the records from the tree (node_c, FP64 CoM and centre) and the
+ * quadrupoles; a32 keeps {CoM, m'} for the particles' offsets */
 __global__ void k_prep32_nodes(size_t bound, const int *__restrict__ nn,
 		const double4 *__restrict__ na,
-		const double4 *__restrict__ nb, double o0, double o1, double o2,
-		double S, double mscale, int4 *__restrict__ a32,
-		int4 *__restrict__ b32)
+		const double4 *__restrict__ nb, const int4 *__restrict__ nc,
+		const float4 *__restrict__ q32, int quad, double o0, double o1,
+		double o2, double S, double mscale, int4 *__restrict__ a32,
+		Node32 *__restrict__ rec)
 {
 	const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (j >= bound || (long long)j >= (long long)*nn)
@@ -101,11 +113,23 @@ __global__ void k_prep32_nodes(size_t bound, const int *__restrict__ nn,
 			(int)fixp(a.z, o2, S), __float_as_int((float)(a.w * mscale)));
 	a32[j] = A;
 	const float s = (float)(b.w * S);
-	b32[j] = make_int4(
+	Node32 r;
+	r.a = A;
+	r.b = make_int4(
 			__float_as_int((float)(int)(fixp(b.x, o0, S) - (unsigned)A.x)),
 			__float_as_int((float)(int)(fixp(b.y, o1, S) - (unsigned)A.y)),
 			__float_as_int((float)(int)(fixp(b.z, o2, S) - (unsigned)A.z)),
 			__float_as_int(s));
+	const int4 c = nc[j];
+	float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0;
+	if (quad) {
+		q0 = q32[2 * j];
+		q1 = q32[2 * j + 1];
+	}
+	r.c = make_int4(c.x, c.y, __float_as_int(q0.x), __float_as_int(q0.y));
+	r.q = make_int4(__float_as_int(q0.z), __float_as_int(q0.w),
+			__float_as_int(q1.x), 0);
+	rec[j] = r;
 }
 
 __global__ void k_prep32_top(const TopNodes T, double o0, double o1,
@@ -751,14 +775,13 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 		double4 *__restrict__ partial, int nslot, int nact,
 		const int *__restrict__ active, const int4 *__restrict__ p32,
 		const double *__restrict__ last_acc, TopNodes T,
-		const int4 *__restrict__ a32, const int4 *__restrict__ b32,
-		const int4 *__restrict__ nc, const float4 *__restrict__ tab,
+		const Node32 *__restrict__ rec, const float4 *__restrict__ tab,
 		Walk32Params w, double *__restrict__ ax, double *__restrict__ ay,
 		double *__restrict__ az, double *__restrict__ gx,
 		double *__restrict__ gy, double *__restrict__ gz,
 		double *__restrict__ cost, unsigned long long *counter,
-		const int4 *__restrict__ top32, const float4 *__restrict__ q32,
-		const float4 *__restrict__ tq32, const float4 *__restrict__ p32o)
+		const int4 *__restrict__ top32, const float4 *__restrict__ tq32,
+		const float4 *__restrict__ p32o)
 {
 	if (*T.nnodes < 0)                   /* the build overflowed: no walk */
 		return;
@@ -830,7 +853,8 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 		while (node < lo) {
 			/* internal nodes have DNext > 0, the root's reaching the
 			 * terminator (tree_build.c:466-491) */
-			const int4 c = nc[node];
+			const Node32 R = rec[node];
+			const int4 c = R.c;
 			if (c.x < 0) {
 				node++;
 				continue;
@@ -844,8 +868,8 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 				node = skip;
 				continue;
 			}
-			const int4 A = a32[node];
-			const int4 B = b32[node];
+			const int4 A = R.a;
+			const int4 B = R.b;
 			const float m = __int_as_float(A.w), s = __int_as_float(B.w);
 			const float rx = fdiff(A.x, me.x), ry = fdiff(A.y, me.y),
 					rz = fdiff(A.z, me.z);
@@ -866,18 +890,24 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 		for (;;) {
 			if (node >= hi)
 				break;
-			const int4 c = nc[node];
+			const Node32 R = rec[node];
+			const int4 c = R.c;
 			if (c.x == 0 && node != start)
 				break;
 			const bool pending = node >= ew_until;
+			/* the quadrupole {Txx, Tyy, Txy, Txz}, Tyz */
+			const float4 q0 = make_float4(__int_as_float(c.z),
+					__int_as_float(c.w), __int_as_float(R.q.x),
+					__int_as_float(R.q.y));
+			const float qyz = __int_as_float(R.q.z);
+			const int4 A = R.a;
+			const int4 B = R.b;
 			if (c.x < 0) {
 				const int f = -c.x - 1, l = f + c.y;
-				const int4 A = a32[node];
 				if (c.y >= w.bundle_min) {
 					/* a bundle that no lane's criterion opens is taken
 					 * whole, like a node: its multipoles at its centre of
 					 * mass instead of c.y particle interactions */
-					const int4 B = b32[node];
 					const float m = __int_as_float(A.w),
 							s = __int_as_float(B.w);
 					const float rx = fdiff(A.x, me.x), ry = fdiff(A.y, me.y),
@@ -887,12 +917,6 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 					if (!__any(node_opens_ri(m, s, r2, ri,
 							rx + __int_as_float(B.x), ry + __int_as_float(B.y),
 							rz + __int_as_float(B.z), crit, w))) {
-						float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f);
-						float qyz = 0.f;
-						if (w.quad) {
-							q0 = q32[2 * node];
-							qyz = q32[2 * node + 1].x;
-						}
 						node32(a, m, q0, qyz, rx, ry, rz, ri, w);
 						if (pending) {
 							if (w.quad)
@@ -913,7 +937,6 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 				if (pending) {
 					/* the bundle's correction at its CoM to second order when
 					 * every lane allows it, else per particle */
-					const int4 B = b32[node];
 					const float s = __int_as_float(B.w),
 							m = __int_as_float(A.w);
 					const float rx = fdiff(A.x, me.x), ry = fdiff(A.y, me.y),
@@ -924,8 +947,7 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 					com_frame = __all(ok);
 					if (com_frame) {
 						if (w.quad)
-							ewald16q(a, tab, m, q32[2 * node],
-									q32[2 * node + 1].x, rx, ry, rz, w);
+							ewald16q(a, tab, m, q0, qyz, rx, ry, rz, w);
 						else
 							ewald16(a, tab, m, rx, ry, rz, w);
 					}
@@ -988,8 +1010,6 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 				node++;
 				continue;
 			}
-			const int4 A = a32[node];
-			const int4 B = b32[node];
 			const float m = __int_as_float(A.w), s = __int_as_float(B.w);
 			const float rx = fdiff(A.x, me.x), ry = fdiff(A.y, me.y),
 					rz = fdiff(A.z, me.z);
@@ -999,12 +1019,6 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 					sy = ry + __int_as_float(B.y),
 					sz = rz + __int_as_float(B.z);
 			if (!__any(node_opens_ri(m, s, r2, ri, sx, sy, sz, crit, w))) {
-				float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f);
-				float qyz = 0.f;
-				if (w.quad) {
-					q0 = q32[2 * node];
-					qyz = q32[2 * node + 1].x;
-				}
 				node32(a, m, q0, qyz, rx, ry, rz, ri, w);
 				if (pending) {
 					if (w.quad)
@@ -1019,8 +1033,7 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 			if (pending && __all(ewald_ok(m, s, rx, ry, rz, sx, sy, sz, crit,
 					false, w))) {
 				if (w.quad)
-					ewald16q(a, tab, m, q32[2 * node], q32[2 * node + 1].x, rx,
-							ry, rz, w);
+					ewald16q(a, tab, m, q0, qyz, rx, ry, rz, w);
 				else
 					ewald16(a, tab, m, rx, ry, rz, w);
 				ew_until = node + c.x;
@@ -1059,18 +1072,18 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
  * compiler's own); the autotuner picks W per workload. */
 #define TDV_WALK32_ARGS int nact, const int *__restrict__ active, \
 		const int4 *__restrict__ p32, const double *__restrict__ last_acc, \
-		TopNodes T, const int4 *__restrict__ a32, const int4 *__restrict__ b32, \
-		const int4 *__restrict__ nc, const float4 *__restrict__ tab, \
+		TopNodes T, const Node32 *__restrict__ rec, \
+		const float4 *__restrict__ tab, \
 		Walk32Params w, double *__restrict__ ax, double *__restrict__ ay, \
 		double *__restrict__ az, double *__restrict__ gx, \
 		double *__restrict__ gy, double *__restrict__ gz, \
 		double *__restrict__ cost, unsigned long long *counter, \
-		const int4 *__restrict__ top32, const float4 *__restrict__ q32, \
-		const float4 *__restrict__ tq32, const float4 *__restrict__ p32o
+		const int4 *__restrict__ top32, const float4 *__restrict__ tq32, \
+		const float4 *__restrict__ p32o
 #define TDV_WALK32_CALL_PART(g, part, nparts, partial, nslot) \
 		walk32_body(g, part, nparts, partial, nslot, nact, active, p32, \
-		last_acc, T, a32, b32, nc, tab, w, ax, ay, az, gx, gy, gz, cost, \
-		counter, top32, q32, tq32, p32o)
+		last_acc, T, rec, tab, w, ax, ay, az, gx, gy, gz, cost, \
+		counter, top32, tq32, p32o)
 #define TDV_WALK32_CALL(g) TDV_WALK32_CALL_PART(g, 0, 1, nullptr, 0)
 
 __global__ void __launch_bounds__(64) k_walk32(TDV_WALK32_ARGS)
@@ -1267,8 +1280,8 @@ int tdv_walk32(tdv_context *c, int use_bh)
 		tdv_graph_drop(c);               /* the records move */
 		if (c->a32)
 			(void)hipFree(c->a32);
-		if (c->b32)
-			(void)hipFree(c->b32);
+		if (c->rec32)
+			(void)hipFree(c->rec32);
 		if (c->q32)
 			(void)hipFree(c->q32);
 		if (c->q_s6)
@@ -1278,11 +1291,12 @@ int tdv_walk32(tdv_context *c, int use_bh)
 		c->q_s6 = nullptr;
 		c->lvl_nodes = nullptr;
 		c->lvl_ok = 0;
-		c->a32 = c->b32 = nullptr;
+		c->a32 = c->rec32 = nullptr;
 		c->q32 = nullptr;
 		c->n32_cap = 0;
 		TDV_HIP(c, hipMalloc((void **)&c->a32, c->node_cap * sizeof(int4)));
-		TDV_HIP(c, hipMalloc((void **)&c->b32, c->node_cap * sizeof(int4)));
+		TDV_HIP(c, hipMalloc((void **)&c->rec32,
+				c->node_cap * sizeof(Node32)));
 		TDV_HIP(c, hipMalloc((void **)&c->q32,
 				2 * c->node_cap * sizeof(float4)));
 		TDV_HIP(c, hipMalloc((void **)&c->q_s6,
@@ -1354,7 +1368,8 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	if (bound)
 		hipLaunchKernelGGL(k_prep32_nodes, dim3(tdv_blocks(bound, 256)),
 				dim3(256), 0, c->stream, bound, c->top.nnodes, c->node_a,
-				c->node_b, o[0], o[1], o[2], S, mscale, c->a32, c->b32);
+				c->node_b, c->node_c, c->q32, quad ? 1 : 0, o[0], o[1], o[2],
+				S, mscale, c->a32, reinterpret_cast<Node32 *>(c->rec32));
 	hipLaunchKernelGGL(k_prep32_particles, dim3(tdv_blocks(n, 256)), dim3(256),
 			0, c->stream, n, c->pm, o[0], o[1], o[2], S, mscale,
 			c->tree_parent, c->a32, c->p32, c->p32o);
@@ -1423,9 +1438,9 @@ int tdv_walk32(tdv_context *c, int use_bh)
 				c->stream, c->d_queue, c->d_count);
 		hipLaunchKernelGGL(kern, dim3((unsigned)items), dim3(64), 0,
 				c->stream, c->d_nactive, split, c->partial, 0, c->active,
-				c->p32, c->last_acc, c->top, c->a32, c->b32, c->node_c,
-				c->ewald16, w, c->acc[0], c->acc[1], c->acc[2], c->gacc[0],
-				c->gacc[1], c->gacc[2], c->cost, c->d_count, c->top32, c->q32,
+				c->p32, c->last_acc, c->top,
+				reinterpret_cast<const Node32 *>(c->rec32), c->ewald16, w, c->acc[0], c->acc[1], c->acc[2], c->gacc[0],
+				c->gacc[1], c->gacc[2], c->cost, c->d_count, c->top32,
 				c->tq32, c->p32o);
 		TDV_TRY(tdv_check(c, hipGetLastError(), "k_walk32_items"));
 		hipLaunchKernelGGL(k_walk32_parts_sum, dim3(tdv_blocks(n, 256)),
@@ -1440,10 +1455,10 @@ int tdv_walk32(tdv_context *c, int use_bh)
 				c->stream, c->d_queue, c->d_count);
 		hipLaunchKernelGGL(k_walk32_queue, dim3(TDV_QUEUE_GRID), dim3(64), 0,
 				c->stream, c->d_nactive, c->d_queue, split, c->partial, 0,
-				c->active, c->p32, c->last_acc, c->top, c->a32, c->b32,
-				c->node_c, c->ewald16, w, c->acc[0], c->acc[1], c->acc[2],
+				c->active, c->p32, c->last_acc, c->top,
+				reinterpret_cast<const Node32 *>(c->rec32), c->ewald16, w, c->acc[0], c->acc[1], c->acc[2],
 				c->gacc[0], c->gacc[1], c->gacc[2], c->cost, c->d_count,
-				c->top32, c->q32, c->tq32, c->p32o);
+				c->top32, c->tq32, c->p32o);
 		TDV_TRY(tdv_check(c, hipGetLastError(), "k_walk32_queue"));
 		/* a captured walk serves every active count: always sum (the
 		 * kernel returns at once when the walk did not split) */
@@ -1458,9 +1473,10 @@ int tdv_walk32(tdv_context *c, int use_bh)
 			: c->par.fp32_waves == 8 ? k_walk32_w8 : k_walk32;
 	hipLaunchKernelGGL(kern, dim3(tdv_blocks((size_t)c->nactive, 64)),
 			dim3(64), 0, c->stream, c->nactive, c->active, c->p32, c->last_acc,
-			c->top, c->a32, c->b32, c->node_c, c->ewald16, w, c->acc[0],
+			c->top, reinterpret_cast<const Node32 *>(c->rec32), c->ewald16, w,
+			c->acc[0],
 			c->acc[1], c->acc[2], c->gacc[0], c->gacc[1], c->gacc[2], c->cost,
-			c->d_count, c->top32, c->q32, c->tq32, c->p32o);
+			c->d_count, c->top32, c->tq32, c->p32o);
 	return tdv_check(c, hipGetLastError(), "k_walk32");
 }
 
