@@ -21,8 +21,10 @@ per pair, no table) on 512 sampled particles (the error oracle; the bar is
 p99 <= 2e-3), and against Gravity_Forcetest's form (the same sum with the
 reference's 65^3 table, trilinear) on 8192, whose own error against the
 exact force is reported too.  ``roofline`` covers the dominant kernel,
-timed with HIP events on the engine's stream, counting 20 flops per
-monopole interaction and 60 per Ewald correction.  ``cpu_baseline`` times
+timed with HIP events on the engine's stream, counting the flops each
+interaction's formula takes (FLOPS below: 20 for a point mass, 50 for a
+node with its quadrupole, 60 for an Ewald correction, 120 with the
+quadrupole term), interactions counted on the device.  ``cpu_baseline`` times
 the CPU oracle (the C/OpenMP restatement of the reference path) on the
 host cores: decomposition and build on all particles, the FP64 walks on a
 bounded sample of particles.
@@ -41,6 +43,25 @@ sys.path.insert(0, ROOT)
 PEAK = {"f64": 78.6e12, "f32": 157.3e12, "hbm": 8.0e12}
 FLOPS_PER_INTERACTION = 20.0
 FLOPS_PER_EWALD = 60.0
+# The FP32 walk's interaction kinds and the flops of their formulas:
+#   point mass   d (3), r^2 (5), r^-1 (1), m r^-3 (3), a += f d (6),
+#                the softening test (1), carry share (1)              20
+#   node + quad  d (3), r^2 (5), r^-1 (1), u = d r^-1 (3), T u (15),
+#                u.T u (5), the scalars r^-2, r^-4, g (6),
+#                a += g u - r^-4 T u (12)                              50
+#   Ewald        node lookup (13), second-order expansion (42), m E (6) 60
+#   Ewald + quad the same plus T:S/6 (6 scalings, 18 fma pairs, 6)    120
+FLOPS = {"point": 20.0, "node_quad": 50.0, "ewald": 60.0, "ewald_quad": 120.0}
+COUNTING = ("flops per interaction by kind: 20 point mass, 50 node with "
+            "quadrupole, 60 Ewald correction, 120 Ewald correction with "
+            "quadrupole; interactions counted on the device")
+
+
+def walk32_flops(kinds):
+    """Flops of one FP32 walk from tdv_interaction_kinds."""
+    inter, ewald, nodes, ewq = kinds
+    return (FLOPS["point"] * (inter - nodes) + FLOPS["node_quad"] * nodes
+            + FLOPS["ewald"] * (ewald - ewq) + FLOPS["ewald_quad"] * ewq)
 
 WORKLOADS = {
     "c2": dict(ngrid=128, seed=7, alpha=0.005, desc="periodic Zel'dovich box "
@@ -151,7 +172,7 @@ def measured_traffic(kernel, workload, f32):
     reading), when it was taken on this workload, these settings and this
     kernel source; else None."""
     import hashlib
-    path = os.path.join(ROOT, "profiles", "walk32_c2_traffic.json")
+    path = os.path.join(ROOT, "profiles", "walk32_%s_traffic.json" % workload)
     src = os.path.join(ROOT, "tandav_amd", "csrc", "tdv_walk32.hip")
     try:
         with open(path) as f:
@@ -228,7 +249,7 @@ def cpu_baseline(pos, m, soft, w, sample, gpu_last, gpu_cost):
 # the near-uniform boxes for p99 <= 2e-3 against the exact Ewald sums (2^20
 # particles: p99 2.0e-3 at 0.005, 8e-4 at 0.0025; 2^23: 1.9e-3 at 0.004 on
 # 512 samples, too close to the bar)
-C4 = dict(n=1 << 23, nhalo=256, seed=11, alpha=0.003,
+C4 = dict(n=1 << 23, nhalo=256, seed=11, alpha=0.002,
           desc="clustered periodic box: 256 "
           "Hernquist halos (dN/dM ~ M^-1.9 over 3 decades, r_s ~ M^(1/3)) + "
           "20% uniform background, 2^23 = 8,388,608 particles, 8 time bins")
@@ -331,6 +352,12 @@ def bench_c4(args, f32):
                      "n_active": int(len(act)), "groups": (len(act) + 63) // 64,
                      "eager_ms": t_eager * 1e3, "eager_walk_ms": walk_eager,
                      "graph_ms": t_graph * 1e3, "graph_walk_ms": walk_graph})
+        if b in (0, 4):
+            # the sparse sets' error: the halo cores need the tighter alpha
+            acc_b = e.get("acc")["acc"]
+            pick = np.sort(g.choice(act, min(len(act), 256), replace=False))
+            subs[-1]["err_vs_exact_ewald"] = errors(acc_b[pick],
+                                                    e.forcetest_exact(pick))
         log("c4: bins <= %d, %.4f%% active: eager %.2f ms, graph %.2f ms" % (
             b, 100.0 * len(act) / n, t_eager * 1e3, t_graph * 1e3))
     line = {
@@ -585,9 +612,10 @@ def bench_c5(args, f32):
     t_step = time.perf_counter() - t0
     stage = e.stage_times()
     newton, ewald = e.interaction_counts()
+    kinds = e.interaction_kinds()
     free1 = torch.cuda.mem_get_info()[0]
     walk_s = stage["walk"] / 1e3
-    flops = 20.0 * newton + 60.0 * ewald
+    flops = walk32_flops(kinds)
     line = {
         "metric": "tree-force evals/s per full step (FP32 group walk), "
                   "512^3 box, at p99 |da|/|a| <= 2e-3",
@@ -614,7 +642,7 @@ def bench_c5(args, f32):
                           "achieved": flops / walk_s / 1e12,
                           "peak": 157.3, "unit": "TFLOP/s",
                           "frac": flops / walk_s / 1e12 / 157.3,
-                          "flops_per_interaction": "20 Newton, 60 Ewald"},
+                          "counting": COUNTING},
         "hbm_bytes_per_particle": (free0 - free1) / n,
         "autotune": tune,
         "accuracy": {"vs_table_direct_sum": dict(err_table, n=len(idx)),
@@ -660,6 +688,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the FP32 walk's default launch options "
                          "instead of the autotuner's pick")
+    ap.add_argument("--fp32-options", default=None, metavar="W,Q,S",
+                    help="FP32 walk's launch options (waves, queue, split) "
+                         "instead of the autotuner's pick, e.g. 6,0,0 for a "
+                         "profile of the kernel the autotuner picks")
     ap.add_argument("--only", choices=["fp32", "fp64"], default=None)
     ap.add_argument("--alphas", default=None,
                     help="comma-separated further opening criteria for the "
@@ -786,15 +818,21 @@ def bench_box(args, f32):
         del got
     tune = None
     if args.only != "fp64":
+        fixed = (tuple(int(x) for x in args.fp32_options.split(","))
+                 if args.fp32_options else None)
         e, t_step, stage = run_gpu(pos, m, soft, w, args.steps, args.warmup,
-                                   "fp32", f32, tune=not args.no_autotune)
+                                   "fp32", f32,
+                                   tune=not args.no_autotune and not fixed,
+                                   opts=fixed)
         err, err_t = score(e)
         newton, ewald = e.interaction_counts()
+        kinds = e.interaction_kinds()
         tune = e.autotune
         pe = e.params
         opts = (int(pe.fp32_waves), int(pe.fp32_queue), int(pe.fp32_split))
         res["fp32"] = dict(t_step=t_step, stage=stage, err=err,
                            err_table=err_t, newton=newton, ewald=ewald,
+                           kinds=kinds,
                            steps=args.steps, nnodes=e.num_nodes(),
                            kernel=walk32_kernel(*opts, n), opts=opts)
         log("fp32: %.1f ms/step, err vs exact %s" % (t_step * 1e3, err))
@@ -820,8 +858,7 @@ def bench_box(args, f32):
     head = res["fp32"] if "fp32" in res else res["fp64"]
     if "fp32" in res:
         kern, t_k = head["kernel"], head["stage"]["walk"]
-        flops = (head["newton"] * FLOPS_PER_INTERACTION
-                 + head["ewald"] * FLOPS_PER_EWALD)
+        flops = walk32_flops(head["kinds"])
         peak, dtype = PEAK["f32"], "f32"
     else:
         walk_ms, ewald_ms = head["stage"]["walk"], head["stage"]["ewald"]
@@ -838,8 +875,9 @@ def bench_box(args, f32):
             "traffic": measured_traffic(kern, args.workload, f32),
             "avg_launch_ms": t_k,
             "flops_per_launch": flops,
-            "counting": "20 flops per monopole interaction, 60 per Ewald "
-                        "correction; interactions counted on the device"}
+            "counting": (COUNTING if "fp32" in res else
+                         "20 flops per monopole interaction, 60 per Ewald "
+                         "correction; interactions counted on the device")}
 
     def summary(r):
         walk_s = (r["stage"]["walk"] + r["stage"]["ewald"]) * 1e-3

@@ -315,6 +315,13 @@ int tdv_stage_times(tdv_context *ctx, float ms[8]);
  * (interact_with_ewald_cube, Gravity/tree_periodic.c:437). */
 int tdv_interaction_counts(tdv_context *ctx, uint64_t counts[2]);
 
+/* The same two sums and, for the FP32 walk, the parts of them that took
+ * more than a point mass: counts[2] of the tree interactions were nodes or
+ * bundles taken with their quadrupole, counts[3] of the Ewald corrections
+ * carried the quadrupole term (zero for the FP64 walk).  An extension for
+ * the flop count of the FP32 walk; no counterpart in the reference. */
+int tdv_interaction_kinds(tdv_context *ctx, uint64_t counts[4]);
+
 /* P.Key as 2n uint64 words: the standard keys of the last sort
  * (reversed = 0, what Sort_Particles_By_Peano_Key leaves) or the reversed
  * keys (reversed = 1, what Reverse_Peano_Keys leaves, peano.c:84-97). */

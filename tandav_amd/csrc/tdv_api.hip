@@ -335,7 +335,7 @@ int tdv_create(const tdv_params *p, tdv_context **out)
 	if (!r)
 		r = dalloc(c, &c->d_err, 1);
 	if (!r)
-		r = dalloc(c, &c->d_count, 2);
+		r = dalloc(c, &c->d_count, 4);
 	if (!r)
 		r = dalloc(c, &c->d_nactive, 1);
 	if (!r)
@@ -799,7 +799,7 @@ int tdv_tree_acceleration(tdv_context *c, int use_bh)
 {
 	if (!c || !c->have_tree)
 		return tdv_fail(c, TDV_ERR_ARG, "walk before a tree build");
-	TDV_HIP(c, hipMemsetAsync(c->d_count, 0, 2 * sizeof(unsigned long long),
+	TDV_HIP(c, hipMemsetAsync(c->d_count, 0, 4 * sizeof(unsigned long long),
 			c->stream));
 	(void)hipEventRecord(c->ev[EV_WALK0], c->stream);
 	if (c->par.precision == TDV_WALK_FP32) {
@@ -1453,6 +1453,15 @@ int tdv_interaction_counts(tdv_context *c, uint64_t counts[2])
 		return TDV_ERR_ARG;
 	counts[0] = c->count[0];
 	counts[1] = c->count[1];
+	return 0;
+}
+
+extern "C" int tdv_interaction_kinds(tdv_context *c, uint64_t counts[4])
+{
+	if (!c || !counts)
+		return TDV_ERR_ARG;
+	for (int k = 0; k < 4; k++)
+		counts[k] = c->count[k];
 	return 0;
 }
 

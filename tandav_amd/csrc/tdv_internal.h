@@ -143,8 +143,10 @@ struct tdv_context {
 	int ewald16_ok;                  /* ewald16 matches ewald */
 
 	unsigned *d_err;
-	unsigned long long *d_count;     /* [2] interactions: tree, Ewald */
-	unsigned long long count[2];
+	unsigned long long *d_count;     /* [4] interactions: tree, Ewald;
+	                                    FP32 walk: of which multipole nodes,
+	                                    corrections with quadrupoles */
+	unsigned long long count[4];
 	int have_domain, have_tree;
 	int have_domain_set, have_top;
 

@@ -450,7 +450,9 @@ struct Acc32 {
 	float x, y, z;
 	double X, Y, Z;
 	int since;
-	unsigned long long cost, ewald;
+	unsigned long long cost, ewald;  /* interactions, Ewald corrections */
+	unsigned nodes, ewq;             /* of which multipole nodes, and
+	                                    corrections with quadrupoles */
 };
 
 __device__ __forceinline__ void carry(Acc32 &a)
@@ -649,6 +651,7 @@ __device__ __forceinline__ void ewald16q(Acc32 &a, const float4 *__restrict__ t,
 	a.y = fmaf(c.sy, vy, a.y);
 	a.z = fmaf(c.sz, vz, a.z);
 	a.ewald++;
+	a.ewq++;
 }
 
 __device__ __forceinline__ float fdiff(int a, int b)
@@ -738,6 +741,7 @@ __device__ __forceinline__ void node32(Acc32 &a, float m, float4 qa,
 		const Walk32Params &w)
 {
 	a.cost++;
+	a.nodes++;
 	const float ri2 = ri * ri;
 	if (!w.quad) {
 		const float fac = m * ri * ri2;
@@ -793,7 +797,7 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 	/* relative criterion: open if M s^2 > r^4 |a| alpha / G; with masses
 	 * scaled by G S^2 and lengths in fixed units this is m s^2 > r^4 crit */
 	const float crit = (float)(last_acc[ip] * (double)w.alpha_rel);
-	Acc32 a = { 0.f, 0.f, 0.f, 0.0, 0.0, 0.0, 0, 0ull, 0ull };
+	Acc32 a = { 0.f, 0.f, 0.f, 0.0, 0.0, 0.0, 0, 0ull, 0ull, 0u, 0u };
 	const int every = w.carry_every;
 
 	for (int j = 0; j < ntop; j++) {
@@ -1055,16 +1059,23 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 		gz[ip] = a.Z;
 		cost[ip] = (double)(a.cost + a.ewald);
 	}
-	unsigned long long v = live ? a.cost : 0ull, e = live ? a.ewald : 0ull;
+	unsigned long long v = live ? a.cost : 0ull, e = live ? a.ewald : 0ull,
+			nd = live ? a.nodes : 0ull, eq = live ? a.ewq : 0ull;
 	for (int d = 32; d >= 1; d >>= 1) {
 		v += __shfl_xor(v, d, 64);
 		e += __shfl_xor(e, d, 64);
+		nd += __shfl_xor(nd, d, 64);
+		eq += __shfl_xor(eq, d, 64);
 	}
 	if (threadIdx.x == 0) {
 		if (v)
 			atomicAdd(counter, v);
 		if (e)
 			atomicAdd(counter + 1, e);
+		if (nd)
+			atomicAdd(counter + 2, nd);
+		if (eq)
+			atomicAdd(counter + 3, eq);
 	}
 }
 
@@ -1228,7 +1239,7 @@ __global__ void k_walk32_queue_reset(unsigned *__restrict__ queue,
 {
 	if (threadIdx.x == 0)
 		queue[0] = 0;
-	if (threadIdx.x < 2)
+	if (threadIdx.x < 4)
 		counter[threadIdx.x] = 0;
 }
 

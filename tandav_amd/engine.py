@@ -146,6 +146,8 @@ def load_library(path=LIB_PATH):
     L.tdv_stage_times.argtypes = [_ctx, ctypes.POINTER(ctypes.c_float)]
     L.tdv_interaction_counts.argtypes = [_ctx,
                                          ctypes.POINTER(ctypes.c_uint64)]
+    L.tdv_interaction_kinds.argtypes = [_ctx,
+                                        ctypes.POINTER(ctypes.c_uint64)]
     L.tdv_get_keys.argtypes = [_ctx, _vp, ctypes.c_int]
     L.tdv_permute_host_columns.argtypes = [_ctx, ctypes.c_int,
                                            ctypes.POINTER(_vp),
@@ -731,6 +733,14 @@ class Engine:
         self._ok(self.lib.tdv_interaction_counts(self._c, c),
                  "tdv_interaction_counts")
         return int(c[0]), int(c[1])
+
+    def interaction_kinds(self):
+        """(tree interactions, Ewald corrections, of those: multipole
+        nodes, corrections with quadrupoles) of the last walk."""
+        c = (ctypes.c_uint64 * 4)()
+        self._ok(self.lib.tdv_interaction_kinds(self._c, c),
+                 "tdv_interaction_kinds")
+        return tuple(int(x) for x in c)
 
     def stage_times(self):
         ms = (ctypes.c_float * 8)()
