@@ -900,7 +900,7 @@ int tdv_capture_step(tdv_context *c, double kick, double drift)
 	if (!c->have_tree || !c->p32 || !c->ewald16_ok || !c->partial
 			|| c->n32_cap < c->node_cap || c->top32_cap < c->top_cap
 			|| c->top_cap < 64 || c->long_cap < long_need
-			|| (c->par.fp32_order != 1 && !c->lvl_nodes))
+			|| !c->lvl_nodes || c->lvl_cap < c->node_cap)
 		return tdv_fail(c, TDV_ERR_ARG, "tdv_capture_step: run one eager "
 				"step (decomposition, FP32 walk) on these particles first");
 	tdv_graph_drop(c);

@@ -26,6 +26,7 @@
  */
 #include "tdv_internal.h"
 #include "tdv_device.h"
+#include <algorithm>
 
 #define NOCOLLAPSE 0xFFFFFFFFu
 
@@ -606,6 +607,169 @@ __global__ void __launch_bounds__(64) k_tree_nodes_long(
 	}
 }
 
+/* The FP32 walk's build: the same nodes, links and parent pointers as
+ * k_tree_nodes/_long, with no scan along the chains.  Every node of a chain
+ * is the cell of its level that starts at the chain's head particle i, so
+ * its end is a binary search over the sorted keys (the last particle whose
+ * key shares the cell with key_i); the head's own parent (a level-(lo-1)
+ * cell, owned by the particle that starts it) is found the same way
+ * backwards.  The centres of mass come afterwards, bottom-up over the
+ * levels (k_com_level): a sum in tree order instead of the reference's
+ * insertion order, which the FP64 walk keeps bit for bit and the FP32 walk
+ * does not need. */
+
+/* the last particle in [a, b] whose key shares the level-lv cell with ki
+ * (particle a does) */
+__device__ __forceinline__ int cell_end(const uint64_t *key, Key2 ki, int a,
+		int b, int lv)
+{
+	while (a < b) {
+		const int mid = a + ((b - a + 1) >> 1);
+		if (div_level(ki, load_key(key, mid)) > lv)
+			a = mid;
+		else
+			b = mid - 1;
+	}
+	return a;
+}
+
+/* the first particle in [a, b] whose key shares the level-lv cell with kb
+ * (particle b does) */
+__device__ __forceinline__ int cell_start(const uint64_t *key, Key2 kb, int a,
+		int b, int lv)
+{
+	while (a < b) {
+		const int mid = a + ((b - a) >> 1);
+		if (div_level(load_key(key, mid), kb) > lv)
+			b = mid;
+		else
+			a = mid + 1;
+	}
+	return b;
+}
+
+__global__ void __launch_bounds__(256) k_tree_links(size_t n,
+		const uint64_t *__restrict__ key, TopNodes T, int vs,
+		const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ off,
+		const int *__restrict__ lohi, int4 *__restrict__ nc,
+		int *__restrict__ up, int *__restrict__ tree_parent,
+		unsigned *err)
+{
+	const size_t gi = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (gi >= n)
+		return;
+	const int i = (int)gi;
+	if (cnt[i] == 0 || *T.nnodes < 0)
+		return;
+	const TopRange r = top_of(key, gi, T, vs);
+	const int lo = lohi[i] & 0xFF, hi = (lohi[i] >> 8) & 0xFF;
+	const int base = node_index(T, r.t, r.first, off, i, r.last);
+	const Key2 ki = load_key(key, i);
+	int e = r.last;
+	for (int lv = lo; lv <= hi; lv++) {
+		e = cell_end(key, ki, i, e, lv);
+		const int kn = e + 1;
+		const int j = base + (lv - lo);
+		const bool leaf = lv == hi;
+		int bits = lv | ((int)bitfield_triplet(ki, lv) << 6)
+				| (1 << TDV_BIT_LOCAL);
+		const bool root = j == T.target[r.t];
+		if (root)
+			bits |= 1 << TDV_BIT_TOP;
+		const int next = node_index(T, r.t, r.first, off, kn, r.last);
+		nc[j] = make_int4(leaf ? (-i - 1) : (next - j), kn - i, bits, i);
+		if (root) {
+			up[j] = r.t;
+		} else if (lv > lo) {
+			up[j] = 1;
+		} else {
+			/* the parent: the level-(lo-1) cell around i, a node of the
+			 * chain its first particle heads */
+			const int p0 = cell_start(key, ki, r.first, i, lo - 1);
+			const int lp = lohi[p0] & 0xFF, hp = (lohi[p0] >> 8) & 0xFF;
+			if (cnt[p0] == 0 || lp > lo - 1 || hp < lo - 1) {
+				atomicOr(err, TDV_EBIT_RANGE);
+				return;
+			}
+			const int jp = node_index(T, r.t, r.first, off, p0, r.last)
+					+ (lo - 1 - lp);
+			up[j] = j - jp;
+		}
+		if (leaf)
+			for (int q = i; q < kn; q++)
+				tree_parent[q] = j;
+		else if (lv == TDV_NT - 1)           /* the dumped particles' node */
+			for (int q = i + 1; q < kn; q++)
+				tree_parent[q] = j;
+	}
+}
+
+/* the particles of [f, f + count) that lane `sub` of eight takes: f + sub,
+ * f + sub + 8, ... into m, m x, m y, m z */
+__device__ __forceinline__ void com_add_strided(const double *__restrict__ x,
+		const double *__restrict__ y, const double *__restrict__ z,
+		const double *__restrict__ m, int f, int count, int sub, double *s)
+{
+	for (int p = f + sub; p < f + count; p += 8) {
+		const double mp = m[p];
+		s[0] += mp;
+		s[1] += mp * x[p];
+		s[2] += mp * y[p];
+		s[3] += mp * z[p];
+	}
+}
+
+/* Centres of mass and masses of one level's nodes, eight lanes per node,
+ * from the children (one level deeper, done by the launch before) and the
+ * particles no child covers; a fixed grid strides over the level's list. */
+__global__ void __launch_bounds__(256) k_com_level(int lv,
+		const unsigned *__restrict__ lcnt, const int *__restrict__ loff,
+		const int *__restrict__ list, const int4 *__restrict__ nc,
+		const double *__restrict__ x, const double *__restrict__ y,
+		const double *__restrict__ z, const double *__restrict__ m,
+		double4 *__restrict__ na)
+{
+	const int count = (int)lcnt[lv];
+	const int base = loff[lv];
+	for (int t0 = (blockIdx.x * blockDim.x) >> 3; t0 < count;
+			t0 += (gridDim.x * blockDim.x) >> 3) {
+		const int t = t0 + (threadIdx.x >> 3), sub = threadIdx.x & 7;
+		const bool on = t < count;
+		double s[4] = { 0, 0, 0, 0 };
+		int j = 0;
+		if (on) {
+			j = list[base + t];
+			const int4 c = nc[j];
+			if (c.x > 1) {
+				int covered = c.w, k = 0;
+				const int end = j + c.x;
+				for (int ch = j + 1; ch < end; k++) {
+					const int4 cc = nc[ch];
+					com_add_strided(x, y, z, m, covered, cc.w - covered, sub, s);
+					if ((k & 7) == sub) {
+						const double4 a = na[ch];
+						s[0] += a.w;
+						s[1] += a.w * a.x;
+						s[2] += a.w * a.y;
+						s[3] += a.w * a.z;
+					}
+					covered = cc.w + cc.y;
+					ch = cc.x > 0 ? ch + cc.x : ch + 1;
+				}
+				com_add_strided(x, y, z, m, covered, c.w + c.y - covered, sub,
+						s);
+			} else {
+				com_add_strided(x, y, z, m, c.w, c.y, sub, s);
+			}
+		}
+		for (int k = 0; k < 4; k++)
+			for (int d = 4; d >= 1; d >>= 1)
+				s[k] += __shfl_xor(s[k], d, 8);
+		if (on && sub == 0)
+			na[j] = make_double4(s[1] / s[0], s[2] / s[0], s[3] / s[0], s[0]);
+	}
+}
+
 __global__ void k_tree_long_reset(unsigned *__restrict__ long_count,
 		unsigned *__restrict__ queue)
 {
@@ -766,22 +930,46 @@ int tdv_build_tree(tdv_context *c)
 		TDV_HIP(c, hipMalloc((void **)&c->long_list, long_need * sizeof(int2)));
 		c->long_cap = long_need;
 	}
-	hipLaunchKernelGGL(k_tree_long_reset, dim3(1), dim3(64), 0, c->stream,
-			c->long_count, c->long_count + 1);
-	hipLaunchKernelGGL(k_tree_nodes, dim3(tdv_blocks(n, 256)), dim3(256), 0,
-			c->stream, n, c->key, c->pos[0], c->pos[1], c->pos[2], c->mass,
-			c->top, vs, reinterpret_cast<uint32_t *>(c->cnt), c->off, c->lohi,
-			c->node_a, c->node_c, c->node_up, c->tree_parent, c->long_list,
-			(unsigned)c->long_cap, c->long_count, c->d_err);
-	hipLaunchKernelGGL(k_tree_nodes_long, dim3(TDV_LONG_GRID), dim3(64), 0,
-			c->stream, c->key, c->pos[0], c->pos[1], c->pos[2], c->mass,
-			c->top, vs, reinterpret_cast<uint32_t *>(c->cnt), c->off, c->lohi,
-			c->node_a, c->node_c, c->node_up, c->tree_parent, c->long_list,
-			c->long_count, (unsigned)c->long_cap, c->long_count + 1);
 	const int tgrid = graph ? c->top_cap : c->ntop;
-	hipLaunchKernelGGL(k_tree_terms, dim3(tdv_blocks((size_t)tgrid, 64)),
-			dim3(64), 0, c->stream, c->top, vs, c->node_a, c->node_b, c->node_c,
-			c->node_up);
+	if (c->par.precision == TDV_WALK_FP32) {
+		/* links by binary searches, centres of mass bottom-up */
+		hipLaunchKernelGGL(k_tree_links, dim3(tdv_blocks(n, 256)), dim3(256),
+				0, c->stream, n, c->key, c->top, vs,
+				reinterpret_cast<uint32_t *>(c->cnt), c->off, c->lohi,
+				c->node_c, c->node_up, c->tree_parent, c->d_err);
+		hipLaunchKernelGGL(k_tree_terms, dim3(tdv_blocks((size_t)tgrid, 64)),
+				dim3(64), 0, c->stream, c->top, vs, c->node_a, c->node_b,
+				c->node_c, c->node_up);
+		if (bound > 0) {
+			TDV_TRY(tdv_level_lists(c, bound));
+			const unsigned grid = (unsigned)std::min(
+					(size_t)tdv_blocks(8 * bound, 256), (size_t)2048);
+			for (int lv = TDV_NT; lv >= 0; lv--)
+				hipLaunchKernelGGL(k_com_level, dim3(grid), dim3(256), 0,
+						c->stream, lv, c->q_arrived,
+						reinterpret_cast<const int *>(c->q_arrived + 128),
+						c->lvl_nodes, c->node_c, c->pos[0], c->pos[1],
+						c->pos[2], c->mass, c->node_a);
+		}
+	} else {
+		hipLaunchKernelGGL(k_tree_long_reset, dim3(1), dim3(64), 0,
+				c->stream, c->long_count, c->long_count + 1);
+		hipLaunchKernelGGL(k_tree_nodes, dim3(tdv_blocks(n, 256)), dim3(256),
+				0, c->stream, n, c->key, c->pos[0], c->pos[1], c->pos[2],
+				c->mass, c->top, vs, reinterpret_cast<uint32_t *>(c->cnt),
+				c->off, c->lohi, c->node_a, c->node_c, c->node_up,
+				c->tree_parent, c->long_list, (unsigned)c->long_cap,
+				c->long_count, c->d_err);
+		hipLaunchKernelGGL(k_tree_nodes_long, dim3(TDV_LONG_GRID), dim3(64),
+				0, c->stream, c->key, c->pos[0], c->pos[1], c->pos[2],
+				c->mass, c->top, vs, reinterpret_cast<uint32_t *>(c->cnt),
+				c->off, c->lohi, c->node_a, c->node_c, c->node_up,
+				c->tree_parent, c->long_list, c->long_count,
+				(unsigned)c->long_cap, c->long_count + 1);
+		hipLaunchKernelGGL(k_tree_terms, dim3(tdv_blocks((size_t)tgrid, 64)),
+				dim3(64), 0, c->stream, c->top, vs, c->node_a, c->node_b,
+				c->node_c, c->node_up);
+	}
 	hipLaunchKernelGGL(k_tree_small_top, dim3(tdv_blocks((size_t)tgrid, 64)),
 			dim3(64), 0, c->stream, c->pos[0], c->pos[1], c->pos[2], c->mass,
 			c->top, vs, c->node_a, c->tree_parent);

@@ -85,6 +85,7 @@ struct tdv_context {
 	                                    counters [64, 128), list offsets
 	                                    [128, 192) */
 	int *lvl_nodes;                  /* node indices grouped by level */
+	size_t lvl_cap;                  /* nodes lvl_nodes holds */
 	int lvl_ok;                      /* the lists match the tree */
 	hipGraph_t graph;                /* captured regular-step walk */
 	hipGraphExec_t graph_exec;
@@ -183,6 +184,9 @@ int tdv_scan_u32(tdv_context *c, const uint32_t *in, uint32_t *out,
 int tdv_build_top_nodes(tdv_context *c);
 int tdv_assign_top(tdv_context *c);
 int tdv_build_tree(tdv_context *c);
+/* the tree's per-level node lists (lvl_nodes, counts and offsets in
+ * q_arrived), on the device; sets lvl_ok */
+int tdv_level_lists(tdv_context *c, size_t bound);
 
 /* ---- launchers (tdv_walk64.hip, tdv_ewald.hip) ---- */
 int tdv_walk64(tdv_context *c, int use_bh);

@@ -1243,6 +1243,40 @@ __global__ void k_walk32_queue_reset(unsigned *__restrict__ queue,
 		counter[threadIdx.x] = 0;
 }
 
+/* The per-level node lists, once per build: counts, offsets, fill, all on
+ * the device (the node count is read there too, so a captured step can
+ * rebuild them). */
+int tdv_level_lists(tdv_context *c, size_t bound)
+{
+	if (c->lvl_cap < c->node_cap) {
+		if (c->graph_mode)
+			return tdv_fail(c, TDV_ERR_ARG, "capture: level lists missing");
+		tdv_graph_drop(c);
+		if (c->lvl_nodes)
+			(void)hipFree(c->lvl_nodes);
+		c->lvl_nodes = nullptr;
+		c->lvl_cap = 0;
+		TDV_HIP(c, hipMalloc((void **)&c->lvl_nodes,
+				c->node_cap * sizeof(int)));
+		c->lvl_cap = c->node_cap;
+	}
+	unsigned *cnt = c->q_arrived, *fill = c->q_arrived + 64;
+	int *off = reinterpret_cast<int *>(c->q_arrived + 128);
+	hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(128), 0, c->stream, 128,
+			c->q_arrived);
+	hipLaunchKernelGGL(k_level_count, dim3(tdv_blocks(bound, 256)),
+			dim3(256), 0, c->stream, (int)bound, c->top.nnodes, c->node_c,
+			cnt);
+	hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(64), 0, c->stream, cnt,
+			off, fill);
+	hipLaunchKernelGGL(k_level_fill, dim3(tdv_blocks(bound, 256)), dim3(256),
+			0, c->stream, (int)bound, c->top.nnodes, c->node_c, off, fill,
+			c->lvl_nodes);
+	TDV_HIP(c, hipGetLastError());
+	c->lvl_ok = 1;
+	return 0;
+}
+
 /* the softening length in fixed units: negative (the reference's K1
  * length), and at least half a unit, so the particle itself (r = 0)
  * always takes the finite inner kernel */
@@ -1297,11 +1331,7 @@ int tdv_walk32(tdv_context *c, int use_bh)
 			(void)hipFree(c->q32);
 		if (c->q_s6)
 			(void)hipFree(c->q_s6);
-		if (c->lvl_nodes)
-			(void)hipFree(c->lvl_nodes);
 		c->q_s6 = nullptr;
-		c->lvl_nodes = nullptr;
-		c->lvl_ok = 0;
 		c->a32 = c->rec32 = nullptr;
 		c->q32 = nullptr;
 		c->n32_cap = 0;
@@ -1312,8 +1342,6 @@ int tdv_walk32(tdv_context *c, int use_bh)
 				2 * c->node_cap * sizeof(float4)));
 		TDV_HIP(c, hipMalloc((void **)&c->q_s6,
 				6 * c->node_cap * sizeof(double)));
-		TDV_HIP(c, hipMalloc((void **)&c->lvl_nodes,
-				c->node_cap * sizeof(int)));
 		c->n32_cap = c->node_cap;
 	}
 	const bool quad = c->par.fp32_order != 1;
@@ -1339,23 +1367,8 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	const size_t bound = c->graph_mode ? c->node_cap : nn;
 	const int tgrid = c->graph_mode ? c->top_cap : c->ntop;
 	if (quad) {
-		if (bound && !c->lvl_ok) {
-			/* the tree's per-level node lists, once per build, on the
-			 * device: counts, offsets, fill */
-			unsigned *cnt = c->q_arrived, *fill = c->q_arrived + 64;
-			int *off = reinterpret_cast<int *>(c->q_arrived + 128);
-			hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(128), 0, c->stream,
-					128, c->q_arrived);
-			hipLaunchKernelGGL(k_level_count, dim3(tdv_blocks(bound, 256)),
-					dim3(256), 0, c->stream, (int)bound, c->top.nnodes,
-					c->node_c, cnt);
-			hipLaunchKernelGGL(k_level_offsets, dim3(1), dim3(64), 0,
-					c->stream, cnt, off, fill);
-			hipLaunchKernelGGL(k_level_fill, dim3(tdv_blocks(bound, 256)),
-					dim3(256), 0, c->stream, (int)bound, c->top.nnodes,
-					c->node_c, off, fill, c->lvl_nodes);
-			c->lvl_ok = 1;
-		}
+		if (bound && !c->lvl_ok)
+			TDV_TRY(tdv_level_lists(c, bound));
 		/* bottom-up: a level's nodes take their children's moments */
 		if (bound) {
 			const unsigned grid = (unsigned)std::min(
