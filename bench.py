@@ -360,6 +360,20 @@ def bench_c4(args, f32):
                                                     e.forcetest_exact(pick))
         log("c4: bins <= %d, %.4f%% active: eager %.2f ms, graph %.2f ms" % (
             b, 100.0 * len(act) / n, t_eager * 1e3, t_graph * 1e3))
+    # the substep graph: records of the nodes and the active particles only
+    # (the particles that a substep drifts), quadrupoles kept
+    e.capture_substep_walk()
+    for b, s in enumerate(subs):
+        act = np.flatnonzero(bins <= b).astype(np.int32)
+        e.set_active(act)
+        e.launch_walk()                     # warm
+        e.set_active(act)
+        t0 = time.perf_counter()
+        e.launch_walk()
+        s["substep_graph_ms"] = (time.perf_counter() - t0) * 1e3
+        s["substep_graph_walk_ms"] = e.stage_times()["walk"]
+        log("c4: bins <= %d: substep graph %.2f ms" % (b,
+                                                     s["substep_graph_ms"]))
     line = {
         "metric": "tree-force evals/s per step (FP32 group walk), clustered "
                   "box, and substep time vs active fraction",

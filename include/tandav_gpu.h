@@ -191,6 +191,17 @@ int tdv_capture_walk(tdv_context *ctx);
  * a rebuild). */
 int tdv_launch_walk(tdv_context *ctx);
 
+/* The captured walk for substeps that keep the tree (hierarchical steps:
+ * Drift_To_Sync_Point moves the active particles, drift.c:12-36, and
+ * tdv_tree_update_drift the nodes' centres of mass,
+ * Gravity/tree_update.c): the replay refreshes the walk's node and particle
+ * records but keeps the nodes' quadrupoles of the last eager walk on this
+ * tree, as the reference's tree update keeps every node property but the
+ * drifted centre of mass; the per-level quadrupole pass is most of a small
+ * substep's cost.  Replayed by tdv_launch_walk; replaces a graph
+ * tdv_capture_walk made.  Engine-side; no counterpart in the reference. */
+int tdv_capture_substep_walk(tdv_context *ctx);
+
 /* The whole regular step of a periodic FP32 run as one HIP graph:
  * Domain_Decomposition (domain.c:67-145: PH keys, key sort, reorder, the
  * level-2 top nodes), Compute_Acceleration on a regular step (accel.c:15-71:

@@ -380,6 +380,7 @@ void tdv_destroy(tdv_context *c)
 	dfree(c->p32o);
 	dfree(c->a32);
 	dfree(c->rec32);
+	dfree(c->ctr32);
 	dfree(c->q32);
 	dfree(c->q_s6);
 	dfree(c->q_arrived);
@@ -834,7 +835,7 @@ int tdv_compute_acceleration(tdv_context *c, int prepare_step)
 	return tdv_tree_acceleration(c, 0);
 }
 
-int tdv_capture_walk(tdv_context *c)
+static int capture_walk(tdv_context *c, int substep)
 {
 	if (!c || !c->have_tree || c->par.precision != TDV_WALK_FP32)
 		return tdv_fail(c, TDV_ERR_ARG, "tdv_capture_walk: FP32 walk over a "
@@ -849,9 +850,11 @@ int tdv_capture_walk(tdv_context *c)
 	TDV_HIP(c, hipStreamBeginCapture(c->stream,
 			hipStreamCaptureModeThreadLocal));
 	c->capturing = 1;
+	c->substep_prep = substep;
 	tdv_launch_safe_last_accel_dev(c);
 	int r = tdv_walk32(c, 0);
 	c->capturing = 0;
+	c->substep_prep = 0;
 	hipGraph_t g = nullptr;
 	const hipError_t e = hipStreamEndCapture(c->stream, &g);
 	if (r) {
@@ -864,6 +867,16 @@ int tdv_capture_walk(tdv_context *c)
 	TDV_TRY(tdv_check(c, hipGraphInstantiate(&c->graph_exec, g, nullptr,
 			nullptr, 0), "hipGraphInstantiate"));
 	return 0;
+}
+
+int tdv_capture_walk(tdv_context *c)
+{
+	return capture_walk(c, 0);
+}
+
+extern "C" int tdv_capture_substep_walk(tdv_context *c)
+{
+	return capture_walk(c, 1);
 }
 
 int tdv_launch_walk(tdv_context *c)

@@ -87,6 +87,8 @@ struct tdv_context {
 	int *lvl_nodes;                  /* node indices grouped by level */
 	size_t lvl_cap;                  /* nodes lvl_nodes holds */
 	int lvl_ok;                      /* the lists match the tree */
+	int substep_prep;                /* capturing a substep walk: the
+	                                    quadrupoles and cell centres kept */
 	hipGraph_t graph;                /* captured regular-step walk */
 	hipGraphExec_t graph_exec;
 	int capturing;
@@ -131,6 +133,7 @@ struct tdv_context {
 	/* FP32 group walk: fixed-point records */
 	int4 *p32, *a32;
 	int4 *rec32;                     /* 64-byte node records, 4 int4 each */
+	int4 *ctr32;                     /* node cell centres, fixed units */
 	float4 *p32o;                    /* offset from the leaf's CoM, mass' */
 	size_t n32_cap;
 	float4 *q32;                     /* 2 per node: traceless quadrupole */

@@ -84,6 +84,20 @@ __device__ __forceinline__ unsigned fixp(double x, double o, double S)
 }
 
 
+/* The centre of the level-lv cell that holds fixed-point coordinate u, in
+ * a frame of 2^bits units per domain (32 for the periodic box, 31
+ * otherwise); a cell smaller than one unit is centred on u.  The walk takes
+ * node centres from here, not from the reference's Tree_Node Pos, whose
+ * Node_Size shift wraps below level 30 (Gravity/tree.h; kept bit for bit by
+ * the FP64 build), so deep nodes' Pos can lie far from their particles. */
+__device__ __forceinline__ unsigned cell_centre(unsigned u, int lv, int bits)
+{
+	const int sh = bits - lv;
+	if (sh <= 0)
+		return u;
+	return ((u >> sh) << sh) + (1u << (sh - 1));
+}
+
 /* One node's (or bundle's) walk record, 64 bytes, fetched with one
  * scalar load:
  *   a = {CoM (fixed units), m'}
@@ -96,31 +110,55 @@ struct __align__(64) Node32 {
 	int4 a, b, c, q;
 };
 
-/* the records from the tree (node_c, FP64 CoM and centre) and the
- * quadrupoles; a32 keeps {CoM, m'} for the particles' offsets */
+/* Every node's cell centre in fixed units, from its first particle (the
+ * opening test's centre), when the records are made for a tree; a
+ * substep's records keep them */
+__global__ void k_prep32_centres(size_t bound, const int *__restrict__ nn,
+		const int4 *__restrict__ nc, const double4 *__restrict__ pm,
+		double o0, double o1, double o2, double S, int bits,
+		int4 *__restrict__ ctr)
+{
+	const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= bound || (long long)j >= (long long)*nn)
+		return;
+	const int4 c = nc[j];
+	if (c.w < 0) {                       /* terminator */
+		ctr[j] = make_int4(0, 0, 0, 0);
+		return;
+	}
+	const double4 q = pm[c.w];
+	const int lv = c.z & 63;
+	ctr[j] = make_int4((int)cell_centre(fixp(q.x, o0, S), lv, bits),
+			(int)cell_centre(fixp(q.y, o1, S), lv, bits),
+			(int)cell_centre(fixp(q.z, o2, S), lv, bits), 0);
+}
+
+/* the records from the tree (node_c, FP64 CoM and size) and the
+ * quadrupoles; a32 keeps {CoM, m'} */
 __global__ void k_prep32_nodes(size_t bound, const int *__restrict__ nn,
 		const double4 *__restrict__ na,
 		const double4 *__restrict__ nb, const int4 *__restrict__ nc,
-		const float4 *__restrict__ q32, int quad, double o0, double o1,
-		double o2, double S, double mscale, int4 *__restrict__ a32,
-		Node32 *__restrict__ rec)
+		const float4 *__restrict__ q32, int quad, const int4 *__restrict__ ctr,
+		double o0, double o1, double o2, double S, double mscale,
+		int4 *__restrict__ a32, Node32 *__restrict__ rec)
 {
 	const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (j >= bound || (long long)j >= (long long)*nn)
 		return;
 	const double4 a = na[j], b = nb[j];
+	const int4 c = nc[j];
 	const int4 A = make_int4((int)fixp(a.x, o0, S), (int)fixp(a.y, o1, S),
 			(int)fixp(a.z, o2, S), __float_as_int((float)(a.w * mscale)));
 	a32[j] = A;
 	const float s = (float)(b.w * S);
 	Node32 r;
 	r.a = A;
+	const int4 C = ctr[j];
 	r.b = make_int4(
-			__float_as_int((float)(int)(fixp(b.x, o0, S) - (unsigned)A.x)),
-			__float_as_int((float)(int)(fixp(b.y, o1, S) - (unsigned)A.y)),
-			__float_as_int((float)(int)(fixp(b.z, o2, S) - (unsigned)A.z)),
+			__float_as_int((float)(int)((unsigned)C.x - (unsigned)A.x)),
+			__float_as_int((float)(int)((unsigned)C.y - (unsigned)A.y)),
+			__float_as_int((float)(int)((unsigned)C.z - (unsigned)A.z)),
 			__float_as_int(s));
-	const int4 c = nc[j];
 	float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0;
 	if (quad) {
 		q0 = q32[2 * j];
@@ -189,9 +227,11 @@ __global__ void k_prep32_ewald_taylor(const double *__restrict__ raw,
 /* Every particle's fixed-point record (position, mass') and, in one pass
  * (after the nodes' records), its offset from its bundle's centre of mass
  * in fixed units (an exact integer difference, rounded once to FP32) with
- * its mass', so a group taking the bundle in the centre's frame adds one
- * FP32 add per component.  Particles of small top nodes (no leaf) are not
- * read from the offsets. */
+ * its mass', so a group taking the bundle in the CoM's frame adds one FP32
+ * add per component.  (An offset from the bundle's cell centre, which a
+ * tree update does not move, lost an order of magnitude in accuracy on the
+ * Hernquist sphere: the centre lies farther from the particles.)  Particles
+ * of small top nodes (no leaf) are not read from the offsets. */
 __global__ void k_prep32_particles(size_t n, const double4 *__restrict__ pm,
 		double o0, double o1, double o2, double S, double mscale,
 		const int *__restrict__ leaf, const int4 *__restrict__ a32,
@@ -1314,6 +1354,7 @@ int tdv_walk32(tdv_context *c, int use_bh)
 		c->par.periodic ? 0.0 : c->dom_origin[1],
 		c->par.periodic ? 0.0 : c->dom_origin[2] };
 	const double mscale = c->par.gravity * S * S;
+	const int bits = c->par.periodic ? 32 : 31;    /* fixed units per domain */
 	if (!c->p32) {
 		TDV_HIP(c, hipMalloc((void **)&c->p32, c->cap * sizeof(int4)));
 		TDV_HIP(c, hipMalloc((void **)&c->p32o, c->cap * sizeof(float4)));
@@ -1327,6 +1368,9 @@ int tdv_walk32(tdv_context *c, int use_bh)
 			(void)hipFree(c->a32);
 		if (c->rec32)
 			(void)hipFree(c->rec32);
+		if (c->ctr32)
+			(void)hipFree(c->ctr32);
+		c->ctr32 = nullptr;
 		if (c->q32)
 			(void)hipFree(c->q32);
 		if (c->q_s6)
@@ -1338,6 +1382,8 @@ int tdv_walk32(tdv_context *c, int use_bh)
 		TDV_HIP(c, hipMalloc((void **)&c->a32, c->node_cap * sizeof(int4)));
 		TDV_HIP(c, hipMalloc((void **)&c->rec32,
 				c->node_cap * sizeof(Node32)));
+		TDV_HIP(c, hipMalloc((void **)&c->ctr32,
+				c->node_cap * sizeof(int4)));
 		TDV_HIP(c, hipMalloc((void **)&c->q32,
 				2 * c->node_cap * sizeof(float4)));
 		TDV_HIP(c, hipMalloc((void **)&c->q_s6,
@@ -1366,7 +1412,8 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	 * device) */
 	const size_t bound = c->graph_mode ? c->node_cap : nn;
 	const int tgrid = c->graph_mode ? c->top_cap : c->ntop;
-	if (quad) {
+	/* a substep keeps the quadrupoles of the last eager walk on this tree */
+	if (quad && !c->substep_prep) {
 		if (bound && !c->lvl_ok)
 			TDV_TRY(tdv_level_lists(c, bound));
 		/* bottom-up: a level's nodes take their children's moments */
@@ -1389,11 +1436,16 @@ int tdv_walk32(tdv_context *c, int use_bh)
 		hipLaunchKernelGGL(k_prep32_top, dim3(tdv_blocks((size_t)tgrid, 64)),
 				dim3(64), 0, c->stream, c->top, o[0], o[1], o[2], S, mscale,
 				c->top32);
+	if (bound && !c->substep_prep)
+		hipLaunchKernelGGL(k_prep32_centres, dim3(tdv_blocks(bound, 256)),
+				dim3(256), 0, c->stream, bound, c->top.nnodes, c->node_c,
+				c->pm, o[0], o[1], o[2], S, bits, c->ctr32);
 	if (bound)
 		hipLaunchKernelGGL(k_prep32_nodes, dim3(tdv_blocks(bound, 256)),
 				dim3(256), 0, c->stream, bound, c->top.nnodes, c->node_a,
-				c->node_b, c->node_c, c->q32, quad ? 1 : 0, o[0], o[1], o[2],
-				S, mscale, c->a32, reinterpret_cast<Node32 *>(c->rec32));
+				c->node_b, c->node_c, c->q32, quad ? 1 : 0, c->ctr32, o[0],
+				o[1], o[2], S, mscale, c->a32,
+				reinterpret_cast<Node32 *>(c->rec32));
 	hipLaunchKernelGGL(k_prep32_particles, dim3(tdv_blocks(n, 256)), dim3(256),
 			0, c->stream, n, c->pm, o[0], o[1], o[2], S, mscale,
 			c->tree_parent, c->a32, c->p32, c->p32o);

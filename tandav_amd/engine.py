@@ -164,6 +164,7 @@ def load_library(path=LIB_PATH):
     L.tdv_set_fp32_options.argtypes = [_ctx, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int]
     L.tdv_capture_walk.argtypes = [_ctx]
+    L.tdv_capture_substep_walk.argtypes = [_ctx]
     L.tdv_capture_step.argtypes = [_ctx, ctypes.c_double, ctypes.c_double]
     L.tdv_launch_step.argtypes = [_ctx]
     L.tdv_set_timeline.argtypes = [_ctx, ctypes.POINTER(TdvTimeline)]
@@ -437,6 +438,12 @@ class Engine:
     def capture_walk(self):
         """Capture the regular step's FP32 walk as a HIP graph."""
         self._ok(self.lib.tdv_capture_walk(self._c), "tdv_capture_walk")
+
+    def capture_substep_walk(self):
+        """Capture the FP32 walk of substeps that keep the tree: the walk
+        records refreshed, the quadrupoles of the last eager walk kept."""
+        self._ok(self.lib.tdv_capture_substep_walk(self._c),
+                 "tdv_capture_substep_walk")
 
     def launch_walk(self):
         """Replay the captured walk."""
