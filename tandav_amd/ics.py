@@ -16,6 +16,8 @@ FP64; units are G = 1.
   function dN/dM ~ M^-1.9 over three decades, r_s ~ M^(1/3), uniform
   centres) plus a uniform background.
 """
+import os
+
 import numpy as np
 
 
@@ -41,10 +43,15 @@ def hernquist_sphere(n=65536, seed=1, a=1.0, mass=1.0, rmax_over_a=100.0,
     return pos, vel, m
 
 
+def _fft_workers():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
 def _zeldovich_displacement(ngrid, seed, slope, rms_frac):
+    import scipy.fft
     g = rng(seed)
     white = g.normal(0.0, 1.0, (ngrid, ngrid, ngrid))
-    dk = np.fft.rfftn(white)
+    dk = scipy.fft.rfftn(white, workers=_fft_workers())
     del white
     k1 = np.fft.fftfreq(ngrid) * ngrid          # integer wave numbers
     kz = np.fft.rfftfreq(ngrid) * ngrid
@@ -60,8 +67,8 @@ def _zeldovich_displacement(ngrid, seed, slope, rms_frac):
         inv_k2 = np.where(k2 > 0, 1.0 / k2, 0.0)
     psi = []
     for kd in (kx, ky, kzz):
-        comp = np.fft.irfftn(1j * kd * inv_k2 * dk, s=(ngrid,) * 3,
-                             axes=(0, 1, 2))
+        comp = scipy.fft.irfftn(1j * kd * inv_k2 * dk, s=(ngrid,) * 3,
+                                axes=(0, 1, 2), workers=_fft_workers())
         psi.append(comp.reshape(-1))
     psi = np.stack(psi, 1)
     rms = np.sqrt(np.mean(np.sum(psi * psi, axis=1)))
