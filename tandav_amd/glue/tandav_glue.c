@@ -529,10 +529,39 @@ void Node_Clear(const enum Tree_Bitfield bit, const int node)
 
 #ifdef PERIODIC
 /* Gravity_Periodic_Init (Gravity/periodic.c:180-233): the engine computes
- * the FP64 Ewald table on the GPU when the context is made. */
+ * the FP64 Ewald table on the GPU when the context is made; as the
+ * reference, the table is read from ./ewald_tables.dat when that file
+ * exists (read_ewald_correction_table, :303-323: Fx, Fy, Fz, Fp, each
+ * (N_EWALD+1)^3 Float in box units, raw) and written there otherwise
+ * (write_ewald_correction_table, :325-342). */
+#define EWALD_FILE "./ewald_tables.dat"
+#define EWALD_NODES (65 * 65 * 65)
+
 void Gravity_Periodic_Init(void)
 {
-	context();
+	tdv_context *c = context();
+	double *tab = malloc(4 * (size_t)EWALD_NODES * sizeof(*tab));
+	Assert(tab != NULL, "no memory for the Ewald table");
+	FILE *fp = fopen(EWALD_FILE, "r");
+	if (fp) {
+		const size_t got = fread(tab, sizeof(*tab), 4 * (size_t)EWALD_NODES,
+				fp);
+		fclose(fp);
+		Assert(got == 4 * (size_t)EWALD_NODES, "short read of %s (%zu of %zu "
+				"values)", EWALD_FILE, got, 4 * (size_t)EWALD_NODES);
+		TDV_CHECK(tdv_set_ewald_table(c, tab));
+	} else {
+		TDV_CHECK(tdv_get_ewald_table(c, tab));
+		fp = fopen(EWALD_FILE, "w");
+		if (fp) {
+			const size_t put = fwrite(tab, sizeof(*tab),
+					4 * (size_t)EWALD_NODES, fp);
+			fclose(fp);
+			Assert(put == 4 * (size_t)EWALD_NODES, "short write of %s",
+					EWALD_FILE);
+		}
+	}
+	free(tab);
 }
 
 /* Ewald_Correction (Gravity/periodic.c:35-113), one separation through the

@@ -230,3 +230,34 @@ def test_binding_fp32_walk_accuracy(periodic):
     print(res)
     assert res["p99"] <= 2e-3
     assert res["cost_min"] > 0
+
+
+def test_binding_ewald_table_file(tmp_path, monkeypatch):
+    """Gravity_Periodic_Init (Gravity/periodic.c:180-233, 303-342): without
+    ./ewald_tables.dat it writes the table there (Fx, Fy, Fz, Fp, 65^3
+    doubles each, box units, raw); with the file it takes the table from
+    it.  A file holding twice the table doubles every Ewald_Correction."""
+    import shutil
+    monkeypatch.chdir(tmp_path)
+    pos, vel, m = ics.zeldovich_box(8, seed=2)
+    libs = []
+    for k in range(2):
+        lib = tmp_path / ("libtandav_host_periodic_%d.so" % k)
+        shutil.copy(glue_host.build_host.lib_path(True), lib)
+        libs.append(str(lib))
+    h1 = glue_host.Host(True, path=libs[0])
+    h1.init(pos, vel, m, softening=1 / 320.)
+    f = tmp_path / "ewald_tables.dat"
+    assert f.stat().st_size == 4 * 65 ** 3 * 8
+    written = np.fromfile(f, dtype=np.float64).reshape(4, -1)
+    e = Engine(periodic=True, softening=1e-3)
+    np.testing.assert_array_equal(written.reshape(-1),
+                                  e.ewald_table().reshape(-1))
+    e.close()
+    (2.0 * written).tofile(f)
+    h2 = glue_host.Host(True, path=libs[1])
+    h2.init(pos, vel, m, softening=1 / 320.)
+    g = np.random.Generator(np.random.Philox(9))
+    dr = g.uniform(-0.5, 0.5, (200, 3))
+    np.testing.assert_array_equal(h2.ewald_correction(dr),
+                                  2.0 * h1.ewald_correction(dr))
