@@ -191,7 +191,7 @@ static int alloc_particles(tdv_context *c, size_t n)
 		TDV_TRY(dalloc(c, &c->sk[b], n));
 		TDV_TRY(dalloc(c, &c->sv[b], n));
 	}
-	const size_t nb = (n + 4095) / 4096;
+	const size_t nb = (n + TDV_SORT_TILE - 1) / TDV_SORT_TILE;
 	c->hist_cap = nb * 256 + 1;
 	TDV_TRY(dalloc(c, &c->hist, c->hist_cap));
 	const size_t sp = (c->hist_cap > n + 1 ? c->hist_cap : n + 1) / 1024 + 2;

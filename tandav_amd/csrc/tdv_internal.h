@@ -32,6 +32,9 @@
 #define TDV_BIT_TOP 10
 #define TDV_BIT_UPDATED 11
 #define TDV_MAX_TOP 65536       /* top-node records a context accepts */
+#define TDV_SORT_TILE 6144      /* keys per radix-sort block: 256 threads x
+                                   24 (longer runs per digit store better;
+                                   the LDS staging fits two blocks per CU) */
 
 /* stage events of a context (stage_ms: keys, sort, reorder, top nodes,
  * build, walk, Ewald walk, kick/drift) */

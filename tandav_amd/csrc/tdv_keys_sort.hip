@@ -19,8 +19,8 @@
 #include <mutex>
 
 #define RS_THREADS 256
-#define RS_ITEMS 16
-#define RS_TILE (RS_THREADS * RS_ITEMS)
+#define RS_ITEMS (TDV_SORT_TILE / RS_THREADS)
+#define RS_TILE TDV_SORT_TILE
 #define RS_BINS 256
 #define RS_WAVES (RS_THREADS / 64)
 
@@ -256,19 +256,30 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(
 	h[threadIdx.x] = 0;
 	__syncthreads();
 	const size_t base = (size_t)blockIdx.x * RS_TILE;
-#pragma unroll 4
+	/* every load issued before the first is used: unconditional loads of
+	 * an index clamped into [0, n) (n >= 1), the count only when i < n */
+	uint64_t kk[RS_ITEMS];
+#pragma unroll
 	for (int it = 0; it < RS_ITEMS; it++) {
 		const size_t i = base + (size_t)it * RS_THREADS + threadIdx.x;
-		if (i < n) {
-			const unsigned d = (unsigned)(rs_load(keys, i, stride, off) >> shift)
-					& 255u;
-			atomicAdd(&h[d], 1u);
-		}
+		kk[it] = rs_load(keys, i < n ? i : n - 1, stride, off);
+	}
+#pragma unroll
+	for (int it = 0; it < RS_ITEMS; it++) {
+		const size_t i = base + (size_t)it * RS_THREADS + threadIdx.x;
+		if (i < n)
+			atomicAdd(&h[(unsigned)(kk[it] >> shift) & 255u], 1u);
 	}
 	__syncthreads();
 	hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
+/* One LSD pass's scatter of a tile of RS_TILE keys: every key's rank
+ * among the tile's keys of its digit (stable: waves, items and lanes in
+ * input order), the tile sorted by digit in LDS, then written out in that
+ * order, so consecutive threads store consecutive positions of a digit's
+ * run (one run of ~16 keys per digit and tile) instead of every lane a
+ * key of its own digit somewhere else. */
 __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(
 		const uint64_t *__restrict__ kin, int stride, int off,
 		const uint32_t *__restrict__ vin, uint64_t *__restrict__ kout,
@@ -277,6 +288,10 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(
 {
 	__shared__ uint32_t wcnt[RS_WAVES][RS_BINS];
 	__shared__ uint32_t binoff[RS_BINS];
+	__shared__ uint32_t dstart[RS_BINS];     /* digit's first slot in tile */
+	__shared__ uint32_t wsum[RS_WAVES];
+	__shared__ uint64_t skey[RS_TILE];
+	__shared__ uint32_t sval[RS_TILE];
 	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 	for (int q = 0; q < RS_WAVES; q++)
 		wcnt[q][threadIdx.x] = 0;
@@ -288,12 +303,28 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(
 	uint32_t vv[RS_ITEMS];
 	uint32_t rk[RS_ITEMS];
 	const uint64_t lt = (1ull << lane) - 1;
+	/* all of the wave's loads in flight at once: unconditional, at an
+	 * index clamped into [0, n) (n >= 1); what lies past n is not used */
+#pragma unroll
+	for (int it = 0; it < RS_ITEMS; it++) {
+		const size_t i = base + (size_t)it * 64 + lane;
+		kk[it] = rs_load(kin, i < n ? i : n - 1, stride, off);
+	}
+	if (vin) {
+#pragma unroll
+		for (int it = 0; it < RS_ITEMS; it++) {
+			const size_t i = base + (size_t)it * 64 + lane;
+			vv[it] = vin[i < n ? i : n - 1];
+		}
+	} else {
+#pragma unroll
+		for (int it = 0; it < RS_ITEMS; it++)
+			vv[it] = (uint32_t)(base + (size_t)it * 64 + lane);
+	}
 #pragma unroll
 	for (int it = 0; it < RS_ITEMS; it++) {
 		const size_t i = base + (size_t)it * 64 + lane;
 		const bool valid = i < n;
-		kk[it] = valid ? rs_load(kin, i, stride, off) : 0;
-		vv[it] = valid ? (vin ? vin[i] : (uint32_t)i) : 0;
 		const unsigned d = (unsigned)(kk[it] >> shift) & 255u;
 		uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -310,25 +341,50 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(
 		rk[it] = old + below;
 	}
 	__syncthreads();
-	{
-		uint32_t run = 0;
-		for (int q = 0; q < RS_WAVES; q++) {
-			const uint32_t t = wcnt[q][threadIdx.x];
-			wcnt[q][threadIdx.x] = run;
-			run += t;
-		}
+	/* per digit: the waves' exclusive prefix, and the tile's count, whose
+	 * exclusive scan over the digits places each digit's run in the tile */
+	uint32_t cnt = 0;
+	for (int q = 0; q < RS_WAVES; q++) {
+		const uint32_t t = wcnt[q][threadIdx.x];
+		wcnt[q][threadIdx.x] = cnt;
+		cnt += t;
 	}
+	uint32_t x = cnt;
+#pragma unroll
+	for (int o = 1; o < 64; o <<= 1) {
+		const uint32_t y = __shfl_up(x, o, 64);
+		if (lane >= o)
+			x += y;
+	}
+	if (lane == 63)
+		wsum[w] = x;
+	__syncthreads();
+	uint32_t before = 0;
+	for (int q = 0; q < w; q++)
+		before += wsum[q];
+	dstart[threadIdx.x] = before + x - cnt;
 	__syncthreads();
 #pragma unroll
 	for (int it = 0; it < RS_ITEMS; it++) {
 		const size_t i = base + (size_t)it * 64 + lane;
 		if (i < n) {
 			const unsigned d = (unsigned)(kk[it] >> shift) & 255u;
-			const uint32_t pos = binoff[d] + wcnt[w][d] + rk[it];
-			if (pos < n) {                       /* always, by construction */
-				kout[pos] = kk[it];
-				vout[pos] = vv[it];
-			}
+			const uint32_t slot = dstart[d] + wcnt[w][d] + rk[it];
+			skey[slot] = kk[it];
+			sval[slot] = vv[it];
+		}
+	}
+	__syncthreads();
+	const size_t tile0 = (size_t)blockIdx.x * RS_TILE;
+	const int nvalid = n - tile0 < (size_t)RS_TILE ? (int)(n - tile0)
+			: RS_TILE;
+	for (int j = threadIdx.x; j < nvalid; j += RS_THREADS) {
+		const uint64_t k = skey[j];
+		const unsigned d = (unsigned)(k >> shift) & 255u;
+		const uint32_t pos = binoff[d] + ((uint32_t)j - dstart[d]);
+		if (pos < n) {                       /* always, by construction */
+			kout[pos] = k;
+			vout[pos] = sval[j];
 		}
 	}
 }

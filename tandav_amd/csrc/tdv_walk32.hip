@@ -839,11 +839,13 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 	const float crit = (float)(last_acc[ip] * (double)w.alpha_rel);
 	Acc32 a = { 0.f, 0.f, 0.f, 0.0, 0.0, 0.0, 0, 0ull, 0ull, 0u, 0u };
 	const int every = w.carry_every;
+	const float inv_parts = 1.f / (float)nparts;
 
 	for (int j = 0; j < ntop; j++) {
 		const int4 tc = top32[j];
 		const float tm = __int_as_float(tc.w);
-		const float ts = w.dom_f / (float)(1 << T.level[j]);
+		/* dom_f / 2^level, exact: one ldexp instead of a division */
+		const float ts = ldexpf(w.dom_f, -T.level[j]);
 		const float rx = fdiff(tc.x, me.x), ry = fdiff(tc.y, me.y),
 				rz = fdiff(tc.z, me.z);
 		const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
@@ -886,9 +888,15 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 		const int start = T.target[j];
 		int lo = start, hi = T.term[j];
 		if (nparts > 1) {
-			const long long len = hi - start;
-			lo = start + (int)(len * part / nparts);
-			hi = start + (int)(len * (part + 1) / nparts);
+			/* cut at floor(len p / nparts) in float: monotone in p, 0 at
+			 * p = 0 and below len for p < nparts, so every part computes
+			 * the same cuts and the ranges tile the subtree (a 64-bit
+			 * integer division here cost ~100 scalar instructions per cut,
+			 * twice per top node and part) */
+			const int len = hi - start;
+			lo = start + (int)((float)len * ((float)part * inv_parts));
+			if (part + 1 < nparts)
+				hi = start + (int)((float)len * ((float)(part + 1) * inv_parts));
 			if (lo >= hi)
 				continue;
 		}
@@ -1245,26 +1253,39 @@ __global__ void k_walk32_parts_sum(const int *__restrict__ nact_dev, int split,
 	const int nact = *nact_dev;
 	const int ngroups = (nact + 63) / 64;
 	const int nparts = walk32_parts(ngroups, split);
-	const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-	if (nparts == 1 || gid >= nact)
+	if (nparts == 1)
 		return;
 	const size_t nslot = (size_t)ngroups * 64;
-	double4 a = partial[gid];
-	for (int k = 1; k < nparts; k++) {
-		const double4 b = partial[k * nslot + gid];
-		a.x += b.x;
-		a.y += b.y;
-		a.z += b.z;
-		a.w += b.w;
+	for (int gid = blockIdx.x * blockDim.x + threadIdx.x; gid < nact;
+			gid += gridDim.x * blockDim.x) {
+		double4 a = partial[gid];
+		/* unrolled so that several parts' loads are in flight; the sum
+		 * stays in part order */
+#pragma unroll 8
+		for (int k = 1; k < nparts; k++) {
+			const double4 b = partial[k * nslot + gid];
+			a.x += b.x;
+			a.y += b.y;
+			a.z += b.z;
+			a.w += b.w;
+		}
+		const int ip = active[gid];
+		ax[ip] = ax[ip] + a.x;
+		ay[ip] = ay[ip] + a.y;
+		az[ip] = az[ip] + a.z;
+		gx[ip] = a.x;
+		gy[ip] = a.y;
+		gz[ip] = a.z;
+		cost[ip] = a.w;
 	}
-	const int ip = active[gid];
-	ax[ip] = ax[ip] + a.x;
-	ay[ip] = ay[ip] + a.y;
-	az[ip] = az[ip] + a.z;
-	gx[ip] = a.x;
-	gy[ip] = a.y;
-	gz[ip] = a.z;
-	cost[ip] = a.w;
+}
+
+/* the sum's grid: a walk splits only when its groups x 2 <= TDV_SPLIT_ITEMS,
+ * so that many particles at most (the kernel strides past the grid) */
+static unsigned parts_sum_blocks(size_t n)
+{
+	const size_t most = (size_t)TDV_SPLIT_ITEMS / 2 * 64;
+	return tdv_blocks(n < most ? n : most, 256);
 }
 
 __global__ void k_fill_u32(int n, unsigned *__restrict__ p)
@@ -1519,7 +1540,7 @@ int tdv_walk32(tdv_context *c, int use_bh)
 				c->gacc[1], c->gacc[2], c->cost, c->d_count, c->top32,
 				c->tq32, c->p32o);
 		TDV_TRY(tdv_check(c, hipGetLastError(), "k_walk32_items"));
-		hipLaunchKernelGGL(k_walk32_parts_sum, dim3(tdv_blocks(n, 256)),
+		hipLaunchKernelGGL(k_walk32_parts_sum, dim3(parts_sum_blocks(n)),
 				dim3(256), 0, c->stream, c->d_nactive, split, c->active,
 				c->partial, c->acc[0], c->acc[1], c->acc[2], c->gacc[0],
 				c->gacc[1], c->gacc[2], c->cost);
@@ -1539,7 +1560,7 @@ int tdv_walk32(tdv_context *c, int use_bh)
 		/* a captured walk serves every active count: always sum (the
 		 * kernel returns at once when the walk did not split) */
 		if (parts || c->capturing)
-			hipLaunchKernelGGL(k_walk32_parts_sum, dim3(tdv_blocks(n, 256)),
+			hipLaunchKernelGGL(k_walk32_parts_sum, dim3(parts_sum_blocks(n)),
 					dim3(256), 0, c->stream, c->d_nactive, split, c->active,
 					c->partial, c->acc[0], c->acc[1], c->acc[2], c->gacc[0],
 					c->gacc[1], c->gacc[2], c->cost);

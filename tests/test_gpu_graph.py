@@ -105,6 +105,53 @@ def test_split_walk_matches_whole_walk(case):
         parts.close()
 
 
+@pytest.mark.parametrize("case", ["zeldovich", "clustered", "hernquist"])
+def test_item_split_walk_matches_queue_split_walk(case):
+    """A split walk launched one workgroup per work item (group, part) and
+    the work-queue kernel running the same parts: the same interactions in
+    the same order, so bit-identical results, for the automatic split of a
+    few groups and for forced 3- and 64-way splits of larger sets, eagerly
+    and as graph replays (whose grid is sized for the largest active set)."""
+    if case == "zeldovich":
+        pos, _, m = ics.zeldovich_box(32, seed=7)
+        kw = {}
+    elif case == "clustered":
+        pos, _, m = ics.clustered_box(1 << 16, nhalo=32, seed=11)
+        kw = dict(softening=1 / 2560.)
+    else:
+        pos, _, m = ics.hernquist_sphere(30000, seed=4)
+        kw = dict(periodic=False, softening=0.01)
+    n = len(m)
+    g = np.random.Generator(np.random.Philox(8))
+    for split, sub in ((0, np.sort(g.choice(n, 150, replace=False))),
+                       (3, np.sort(g.choice(n, n // 5, replace=False))),
+                       (64, np.arange(4096, dtype=np.int32))):
+        items = _engine(pos, m, 0, split=split, **kw)
+        queue = _engine(pos, m, 1, split=split, **kw)
+        ref = items.get("pos", "acc", "last_acc_mag")
+        for e in (items, queue):
+            e.set_active(sub)
+            e.update_particles(ref["pos"], acc=ref["acc"],
+                               last_acc_mag=ref["last_acc_mag"])
+            e.compute_acceleration(False)   # as a replay: safe_last_accel
+        a, b = items.get("acc", "cost"), queue.get("acc", "cost")
+        np.testing.assert_array_equal(a["acc"], b["acc"], err_msg=case)
+        np.testing.assert_array_equal(a["cost"], b["cost"], err_msg=case)
+        assert items.interaction_counts() == queue.interaction_counts()
+        want = a
+        items.capture_walk()
+        for rep in range(2):
+            items.update_particles(ref["pos"], acc=ref["acc"],
+                                   last_acc_mag=ref["last_acc_mag"])
+            items.launch_walk()
+            got = items.get("acc", "cost")
+            np.testing.assert_array_equal(got["acc"], want["acc"])
+            np.testing.assert_array_equal(got["cost"], want["cost"])
+            items.get("pos")
+        items.close()
+        queue.close()
+
+
 def test_device_active_list_matches_host_list():
     """Make_Active_Particle_List on the device (Time_Bin <= max bin, in
     particle order) drives the eager walk and a captured graph exactly as
