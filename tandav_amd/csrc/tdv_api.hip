@@ -161,6 +161,8 @@ static void free_particles(tdv_context *c)
 	dfree(c->top.ptop);
 	dfree(c->p32);                   /* FP32 walk records, sized by cap */
 	dfree(c->p32o);
+	dfree(c->gkey);
+	dfree(c->gorder);
 	c->cap = 0;
 }
 
@@ -393,6 +395,8 @@ void tdv_destroy(tdv_context *c)
 	dfree(c->d_nactive);
 	dfree(c->d_queue);
 	dfree(c->partial);
+	dfree(c->gkey);
+	dfree(c->gorder);
 	dfree(c->long_count);
 	dfree(c->long_list);
 	for (int e = 0; e < EV_N; e++)
@@ -911,7 +915,7 @@ int tdv_capture_step(tdv_context *c, double kick, double drift)
 				"finite");
 	const size_t long_need = (c->n / (TDV_LONG_CHAIN + 1) + 1) * (TDV_NT + 2);
 	if (!c->have_tree || !c->p32 || !c->ewald16_ok || !c->partial
-			|| c->n32_cap < c->node_cap || c->top32_cap < c->top_cap
+			|| !c->gorder || c->n32_cap < c->node_cap || c->top32_cap < c->top_cap
 			|| c->top_cap < 64 || c->long_cap < long_need
 			|| !c->lvl_nodes || c->lvl_cap < c->node_cap)
 		return tdv_fail(c, TDV_ERR_ARG, "tdv_capture_step: run one eager "

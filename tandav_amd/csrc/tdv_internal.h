@@ -83,6 +83,9 @@ struct tdv_context {
 	                                    walk and captured graphs read it) */
 	unsigned *d_queue;               /* work-queue counter of the walk */
 	double4 *partial;                /* split walk: TDV_SPLIT_ITEMS x 64 */
+	uint64_t *gkey;                  /* FP32 walk: 2 per group, cost keys */
+	uint32_t *gorder;                /* groups by descending expected cost */
+	size_t gcap;                     /* groups gkey and gorder hold */
 	double *q_s6;                    /* nodes' FP64 second moments, 6 each */
 	unsigned *q_arrived;             /* per level: counts [0, 64), fill
 	                                    counters [64, 128), list offsets
@@ -185,6 +188,8 @@ void tdv_launch_pack_pm(tdv_context *c);
 int tdv_domain_nonperiodic(tdv_context *c);
 int tdv_scan_u32(tdv_context *c, const uint32_t *in, uint32_t *out,
 		size_t n);
+int tdv_radix_pass8(tdv_context *c, size_t n, const uint64_t *keys,
+		uint64_t *kout, uint32_t *order);
 
 /* ---- launchers (tdv_build.hip) ---- */
 int tdv_build_top_nodes(tdv_context *c);
@@ -210,6 +215,9 @@ void tdv_graph_drop(tdv_context *c);
                                             one wavefront each */
 #define TDV_LONG_GRID 2048               /* waves of that kernel */
 #define TDV_SPLIT_AUTO_MAX 64            /* parts per group, split 0 */
+#define TDV_ORDER_MIN_GROUPS 2048        /* FP32 walks of this many whole
+                                            groups or more start the
+                                            costliest first */
 int tdv_direct_sum(tdv_context *c, int nt, const int *idx_dev, double *out_dev);
 int tdv_direct_sum_exact(tdv_context *c, int nt, const int *idx_dev,
 		double *out_dev);

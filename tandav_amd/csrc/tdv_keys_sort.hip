@@ -456,6 +456,24 @@ static int radix_sort_u64(tdv_context *c, const SortWork &w, size_t n,
 	return tdv_check(c, hipGetLastError(), "radix sort");
 }
 
+/* One stable pass over the low 8 bits of n u64 keys with the keys' indices
+ * as payload: work items in key order (the FP32 walk's groups by expected
+ * cost), in index order within a key */
+int tdv_radix_pass8(tdv_context *c, size_t n, const uint64_t *keys,
+		uint64_t *kout, uint32_t *order)
+{
+	const unsigned nb = (unsigned)((n + RS_TILE - 1) / RS_TILE);
+	const size_t hn = (size_t)nb * RS_BINS;
+	if (hn + 1 > c->hist_cap)
+		return tdv_fail(c, TDV_ERR_ARG, "radix pass histogram too small");
+	hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_THREADS), 0, c->stream,
+			keys, 1, 0, n, 0, c->hist, nb);
+	TDV_TRY(scan_u32(c, c->stream, c->scan_part, c->hist, c->hist, hn));
+	hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_THREADS), 0,
+			c->stream, keys, 1, 0, nullptr, kout, order, n, 0, c->hist, nb);
+	return tdv_check(c, hipGetLastError(), "radix pass");
+}
+
 /* order runs of equal high words by the low word; flag duplicate keys */
 __global__ void k_fix_ties(size_t n, const uint64_t *__restrict__ shi,
 		uint32_t *__restrict__ perm, const uint64_t *__restrict__ keys,

@@ -1138,28 +1138,30 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 		double *__restrict__ gy, double *__restrict__ gz, \
 		double *__restrict__ cost, unsigned long long *counter, \
 		const int4 *__restrict__ top32, const float4 *__restrict__ tq32, \
-		const float4 *__restrict__ p32o
+		const float4 *__restrict__ p32o, const uint32_t *__restrict__ gorder
 #define TDV_WALK32_CALL_PART(g, part, nparts, partial, nslot) \
 		walk32_body(g, part, nparts, partial, nslot, nact, active, p32, \
 		last_acc, T, rec, tab, w, ax, ay, az, gx, gy, gz, cost, \
 		counter, top32, tq32, p32o)
 #define TDV_WALK32_CALL(g) TDV_WALK32_CALL_PART(g, 0, 1, nullptr, 0)
+/* the i-th group to walk: by descending expected cost when ordered */
+#define TDV_GROUP(i) (gorder ? (int)gorder[i] : (i))
 
 __global__ void __launch_bounds__(64) k_walk32(TDV_WALK32_ARGS)
 {
-	TDV_WALK32_CALL(blockIdx.x);
+	TDV_WALK32_CALL(TDV_GROUP((int)blockIdx.x));
 }
 
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
 k_walk32_w6(TDV_WALK32_ARGS)
 {
-	TDV_WALK32_CALL(blockIdx.x);
+	TDV_WALK32_CALL(TDV_GROUP((int)blockIdx.x));
 }
 
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8)))
 k_walk32_w8(TDV_WALK32_ARGS)
 {
-	TDV_WALK32_CALL(blockIdx.x);
+	TDV_WALK32_CALL(TDV_GROUP((int)blockIdx.x));
 }
 
 /* Parts per group: 1 when the active groups alone fill the chip, else
@@ -1200,8 +1202,9 @@ __global__ void __launch_bounds__(64) k_walk32_queue(const int *__restrict__ nac
 		const int item = __builtin_amdgcn_readfirstlane((int)t);
 		if (item >= nitems)
 			break;
-		const int g = item / nparts;
-		TDV_WALK32_CALL_PART(g, item - g * nparts, nparts, partial, nslot);
+		const int gi = item / nparts;
+		TDV_WALK32_CALL_PART(TDV_GROUP(gi), item - gi * nparts, nparts,
+				partial, nslot);
 	}
 }
 
@@ -1217,8 +1220,9 @@ __global__ void __launch_bounds__(64) k_walk32_queue(const int *__restrict__ nac
 	const int item = blockIdx.x;                                           \
 	if (item >= ngroups * nparts)                                          \
 		return;                                                            \
-	const int g = item / nparts;                                           \
-	TDV_WALK32_CALL_PART(g, item - g * nparts, nparts, partial, ngroups * 64)
+	const int gi = item / nparts;                                          \
+	TDV_WALK32_CALL_PART(TDV_GROUP(gi), item - gi * nparts, nparts, partial, \
+			ngroups * 64)
 
 __global__ void __launch_bounds__(64) k_walk32_items(
 		const int *__restrict__ nact_dev, int split,
@@ -1286,6 +1290,35 @@ static unsigned parts_sum_blocks(size_t n)
 {
 	const size_t most = (size_t)TDV_SPLIT_ITEMS / 2 * 64;
 	return tdv_blocks(n < most ? n : most, 256);
+}
+
+/* Each group's expected walk time for the launch order: the Cost of its
+ * particles' last walk (interactions), summed by one wave per group, as a
+ * key on a log scale, 8 keys per factor 2, 0 for the costliest; 255 for
+ * the slots past the active groups (they sort last).  Groups then start
+ * longest first and the cheap ones fill the last waves' gaps. */
+__global__ void __launch_bounds__(256) k_group_keys(
+		const int *__restrict__ nact_dev, int ng_max,
+		const int *__restrict__ active, const double *__restrict__ cost,
+		uint64_t *__restrict__ key)
+{
+	const int g = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+	const int lane = threadIdx.x & 63;
+	if (g >= ng_max)
+		return;
+	const int nact = *nact_dev;
+	const int i = g * 64 + lane;
+	double s = i < nact ? cost[active[i]] : 0.0;
+	for (int d = 32; d >= 1; d >>= 1)
+		s += __shfl_xor(s, d, 64);
+	if (lane == 0) {
+		unsigned k = 255u;
+		if (g * 64 < nact) {
+			const float b = 8.f * __log2f(1.f + (float)s);
+			k = 254u - (unsigned)fminf(b, 254.f);
+		}
+		key[g] = k;
+	}
 }
 
 __global__ void k_fill_u32(int n, unsigned *__restrict__ p)
@@ -1379,6 +1412,13 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	if (!c->p32) {
 		TDV_HIP(c, hipMalloc((void **)&c->p32, c->cap * sizeof(int4)));
 		TDV_HIP(c, hipMalloc((void **)&c->p32o, c->cap * sizeof(float4)));
+	}
+	if (!c->gorder) {                    /* the groups' launch order */
+		c->gcap = (c->cap + 63) / 64;
+		TDV_HIP(c, hipMalloc((void **)&c->gkey,
+				2 * c->gcap * sizeof(uint64_t)));
+		TDV_HIP(c, hipMalloc((void **)&c->gorder,
+				c->gcap * sizeof(uint32_t)));
 	}
 	if (!c->partial)                     /* split walks' partial sums */
 		TDV_HIP(c, hipMalloc((void **)&c->partial,
@@ -1524,6 +1564,23 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	const int ngroups_h = (int)((c->nactive + 63) / 64);
 	const int nparts_h = walk32_parts(ngroups_h, split);
 	const bool parts = nparts_h > 1;
+	/* launch order of the groups: longest expected walk first, from the
+	 * Cost of the last walk, for walks of many whole groups (a regular
+	 * step, eager or captured as a whole step); a split walk of few groups
+	 * and a captured walk for any subset keep the list order */
+	const uint32_t *gord = nullptr;
+	const size_t ng_max = (n + 63) / 64;
+	if (c->graph_mode || (!c->capturing && !parts
+			&& ngroups_h >= TDV_ORDER_MIN_GROUPS)) {
+		const size_t ng = c->graph_mode ? ng_max : (size_t)ngroups_h;
+		hipLaunchKernelGGL(k_group_keys, dim3(tdv_blocks(ng, 4)), dim3(256),
+				0, c->stream, c->d_nactive, (int)ng, c->active, c->cost,
+				c->gkey);
+		TDV_TRY(tdv_check(c, hipGetLastError(), "k_group_keys"));
+		TDV_TRY(tdv_radix_pass8(c, ng, c->gkey, c->gkey + c->gcap,
+				c->gorder));
+		gord = c->gorder;
+	}
 	if (!c->par.fp32_queue && (c->capturing || parts)) {
 		/* one workgroup per item; captured: enough for any active set */
 		const size_t items = c->capturing
@@ -1538,7 +1595,7 @@ int tdv_walk32(tdv_context *c, int use_bh)
 				c->p32, c->last_acc, c->top,
 				reinterpret_cast<const Node32 *>(c->rec32), c->ewald16, w, c->acc[0], c->acc[1], c->acc[2], c->gacc[0],
 				c->gacc[1], c->gacc[2], c->cost, c->d_count, c->top32,
-				c->tq32, c->p32o);
+				c->tq32, c->p32o, gord);
 		TDV_TRY(tdv_check(c, hipGetLastError(), "k_walk32_items"));
 		hipLaunchKernelGGL(k_walk32_parts_sum, dim3(parts_sum_blocks(n)),
 				dim3(256), 0, c->stream, c->d_nactive, split, c->active,
@@ -1555,7 +1612,7 @@ int tdv_walk32(tdv_context *c, int use_bh)
 				c->active, c->p32, c->last_acc, c->top,
 				reinterpret_cast<const Node32 *>(c->rec32), c->ewald16, w, c->acc[0], c->acc[1], c->acc[2],
 				c->gacc[0], c->gacc[1], c->gacc[2], c->cost, c->d_count,
-				c->top32, c->tq32, c->p32o);
+				c->top32, c->tq32, c->p32o, gord);
 		TDV_TRY(tdv_check(c, hipGetLastError(), "k_walk32_queue"));
 		/* a captured walk serves every active count: always sum (the
 		 * kernel returns at once when the walk did not split) */
@@ -1573,7 +1630,7 @@ int tdv_walk32(tdv_context *c, int use_bh)
 			c->top, reinterpret_cast<const Node32 *>(c->rec32), c->ewald16, w,
 			c->acc[0],
 			c->acc[1], c->acc[2], c->gacc[0], c->gacc[1], c->gacc[2], c->cost,
-			c->d_count, c->top32, c->tq32, c->p32o);
+			c->d_count, c->top32, c->tq32, c->p32o, gord);
 	return tdv_check(c, hipGetLastError(), "k_walk32");
 }
 

@@ -152,6 +152,31 @@ def test_item_split_walk_matches_queue_split_walk(case):
         queue.close()
 
 
+def test_cost_ordered_walk_matches_list_order():
+    """An eager walk of >= 2048 whole groups launches them by descending
+    Cost of the previous walk; a captured walk graph keeps the list order.
+    Every group runs the same code either way: bit-identical results, with
+    the grid and the work-queue kernel."""
+    pos, _, m = ics.zeldovich_box(64, seed=7)        # 4096 groups
+    for queue in (0, 1):
+        e = _engine(pos, m, queue, split=1)
+        st = e.get("pos", "acc", "last_acc_mag")
+        e.update_particles(st["pos"], acc=st["acc"],
+                           last_acc_mag=st["last_acc_mag"])
+        e.compute_acceleration(False)
+        want = e.get("acc", "cost")
+        assert want["cost"].min() > 0
+        e.capture_walk()
+        for rep in range(2):
+            e.update_particles(st["pos"], acc=st["acc"],
+                               last_acc_mag=st["last_acc_mag"])
+            e.launch_walk()
+            got = e.get("acc", "cost")
+            np.testing.assert_array_equal(got["acc"], want["acc"])
+            np.testing.assert_array_equal(got["cost"], want["cost"])
+        e.close()
+
+
 def test_device_active_list_matches_host_list():
     """Make_Active_Particle_List on the device (Time_Bin <= max bin, in
     particle order) drives the eager walk and a captured graph exactly as
