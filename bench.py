@@ -500,6 +500,50 @@ def bench_domain(args, f32):
     print(json.dumps(line), flush=True)
 
 
+def bench_sort(args):
+    """Domain_Decomposition's keys, sort and reorder (domain.c:67-145,
+    sort.c:253-341, peano.c:50-82) on ngrid^3 uniform random positions in
+    a periodic box (default 512^3 = 134,217,728): each stage's time over
+    --steps decompositions after --warmup, and the sort's algorithmic HBM
+    rate (264 B per particle: 8 passes of 8 B read for the histogram and
+    12 B read + 12 B written by the scatter, the first reading 128-bit keys
+    for their high words)."""
+    from tandav_amd import Engine
+    ng = args.sort_ngrid
+    n = ng ** 3
+    g = np.random.Generator(np.random.Philox(1))
+    pos = g.random((n, 3))
+    m = np.full(n, 1.0 / n)
+    e = Engine(periodic=True, boxsize=1.0, softening=1.0 / (40 * ng),
+               alpha_rel=FP32["alpha"], compat=False, precision="fp32")
+    e.set_particles(pos, m)
+    e.setup_domain()
+    for _ in range(args.warmup):
+        e.domain_decomposition()
+    rows = []
+    for _ in range(args.steps):
+        e.domain_decomposition()
+        st = e.stage_times()
+        rows.append({k: float(st[k]) for k in ("keys", "sort", "reorder")})
+        log("sort: %d keys: keys %.3f ms, sort %.3f ms, reorder %.3f ms"
+            % (n, st["keys"], st["sort"], st["reorder"]))
+    sort_ms = float(np.mean([r["sort"] for r in rows]))
+    line = {
+        "metric": "keys sorted/s (PH keys, LSD radix sort of the high words, "
+                  "inside the domain decomposition)",
+        "value": n / (sort_ms * 1e-3), "unit": "keys/s", "n_gpus": 1,
+        "steps": len(rows), "warmup": args.warmup, "ms_per_step": sort_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "%d^3 = %d uniform random positions, "
+                   "periodic unit box" % (ng, n), "n_particles": n,
+                   "parallelism": "single GPU"},
+        "stages_ms": rows,
+        "sort_algorithmic_GB_s": 264.0 * n / (sort_ms * 1e-3) * 1e-9,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_snapshot(args):
     """Write_Snapshot and Read_Snapshot (IO/write_snapshot.c,
     IO/read_snapshot.c) of the 128^3 box through the C ABI, in a local
@@ -673,7 +717,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c3",
                     choices=sorted(WORKLOADS) + ["c4", "c5", "domain",
-                                                 "snapshot"])
+                                                 "snapshot", "sort"])
+    ap.add_argument("--sort-ngrid", type=int, default=512,
+                    help="the sort workload's ngrid^3 particles")
     ap.add_argument("--ntargets", default="2,8,16,64",
                     help="target counts of the domain workload")
     ap.add_argument("--c4-n", type=int, default=0,
@@ -754,6 +800,9 @@ def main():
         return
     if args.workload == "snapshot":
         bench_snapshot(args)
+        return
+    if args.workload == "sort":
+        bench_sort(args)
         return
 
     bench_box(args, f32)

@@ -216,7 +216,8 @@ void tdv_graph_drop(tdv_context *c);
 #define TDV_LONG_GRID 2048               /* waves of that kernel */
 #define TDV_SPLIT_AUTO_MAX 64            /* parts per group, split 0 */
 #define TDV_ORDER_MIN_GROUPS 2048        /* FP32 walks of this many whole
-                                            groups or more start the
+                                            groups ... */
+#define TDV_ORDER_MAX_GROUPS (1 << 20)   /* ... to this many start the
                                             costliest first */
 int tdv_direct_sum(tdv_context *c, int nt, const int *idx_dev, double *out_dev);
 int tdv_direct_sum_exact(tdv_context *c, int nt, const int *idx_dev,

@@ -1565,14 +1565,20 @@ int tdv_walk32(tdv_context *c, int use_bh)
 	const int nparts_h = walk32_parts(ngroups_h, split);
 	const bool parts = nparts_h > 1;
 	/* launch order of the groups: longest expected walk first, from the
-	 * Cost of the last walk, for walks of many whole groups (a regular
-	 * step, eager or captured as a whole step); a split walk of few groups
-	 * and a captured walk for any subset keep the list order */
+	 * Cost of the last walk, for walks of whole groups that fill the chip
+	 * a few to a few hundred times (a regular step, eager or captured as a
+	 * whole step), where the last groups to start set the end; a split
+	 * walk of few groups and a captured walk for any subset keep the list
+	 * order, and so do walks of more groups, whose neighbours in the list
+	 * then share more of their tree in the caches than the order saves
+	 * (128^3: 88 -> 59 ms, 256^3: 543 -> 524 ms, 512^3 uniform: +0.5%) */
 	const uint32_t *gord = nullptr;
 	const size_t ng_max = (n + 63) / 64;
-	if (c->graph_mode || (!c->capturing && !parts
-			&& ngroups_h >= TDV_ORDER_MIN_GROUPS)) {
-		const size_t ng = c->graph_mode ? ng_max : (size_t)ngroups_h;
+	const size_t ng_walk = c->graph_mode ? ng_max : (size_t)ngroups_h;
+	if ((c->graph_mode || (!c->capturing && !parts))
+			&& ng_walk >= TDV_ORDER_MIN_GROUPS
+			&& ng_walk <= TDV_ORDER_MAX_GROUPS) {
+		const size_t ng = ng_walk;
 		hipLaunchKernelGGL(k_group_keys, dim3(tdv_blocks(ng, 4)), dim3(256),
 				0, c->stream, c->d_nactive, (int)ng, c->active, c->cost,
 				c->gkey);

@@ -206,15 +206,17 @@ def test_device_active_list_matches_host_list():
         b.make_active_list(0)
 
 
-def test_step_graph_matches_eager_steps():
+@pytest.mark.parametrize("ng", [48, 64])
+def test_step_graph_matches_eager_steps(ng):
     """The whole regular step captured as one HIP graph (decomposition,
     build, FP32 walk, kick-drift of one global step) and replayed three
     times, with eager work between the replays, leaves every column, the
-    keys, the top nodes and the tree bit for bit as eager steps do."""
-    pos, vel, m = ics.zeldovich_box(48, seed=7)
+    keys, the top nodes and the tree bit for bit as eager steps do.  At
+    64^3 (4096 groups) both walk their groups by descending Cost."""
+    pos, vel, m = ics.zeldovich_box(ng, seed=7)
     engines = []
     for _ in range(2):
-        e = Engine(periodic=True, softening=1 / 1920., alpha_rel=0.01,
+        e = Engine(periodic=True, softening=1 / (40. * ng), alpha_rel=0.01,
                    precision="fp32", compat=False, ewald_size=0.125,
                    ewald_accuracy=1.0, fp32_order=2, fp32_criterion=3)
         e.set_particles(pos, m, vel=vel * 0.3)
