@@ -248,6 +248,14 @@ int tdv_peano_keys(tdv_context *ctx, size_t n, const double *x,
 		const double *y, const double *z, const double origin[3], double size,
 		int reversed, uint64_t *keys_out);
 
+/* The same keys computed on the host, by the same code as the device's:
+ * no device work, no context, safe to call from any thread at once (the
+ * binding's per-element Peano_Key, Reversed_Peano_Key, peano.c:122-276).
+ * TDV_ERR_ARG for missing arrays or size <= 0. */
+int tdv_peano_keys_host(size_t n, const double *x, const double *y,
+		const double *z, const double origin[3], double size, int reversed,
+		uint64_t *keys_out);
+
 /* Qsort_Index(perm, keys, n, 16, cmp_peanoKeys) (sort.c:90-116): the
  * permutation that sorts unique 128-bit keys ascending; keys = 2n words. */
 int tdv_sort_keys(tdv_context *ctx, size_t n, const uint64_t *keys,
@@ -272,6 +280,14 @@ int tdv_sort_keys_noalloc(size_t n, const uint64_t *keys, uint32_t *perm_out,
  * dr[3n] -> f[3n]. */
 int tdv_ewald_correction(tdv_context *ctx, size_t n, const double *dr,
 		double *f);
+
+/* The same lookup on the host from the context's host copy of the table,
+ * by the same code as the device's: no device work, safe to call from any
+ * thread at once while no call changes the table (the binding's
+ * per-element Ewald_Correction, Gravity/periodic.c:35-113).  TDV_ERR_ARG
+ * without a table. */
+int tdv_ewald_correction_host(const tdv_context *ctx, size_t n,
+		const double *dr, double *f);
 
 /* read/write_ewald_correction_table (Gravity/periodic.c:303-342): the
  * unnormalised 4 x 65^3 table (Fx, Fy, Fz, Fp) in box units. */

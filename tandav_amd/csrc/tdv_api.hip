@@ -1067,6 +1067,19 @@ static int forcetest(tdv_context *c, int nt, const int *idx, double *acc_out,
 	return r;
 }
 
+int tdv_peano_keys_host(size_t n, const double *x, const double *y,
+		const double *z, const double origin[3], double size, int reversed,
+		uint64_t *keys_out)
+{
+	if ((n && (!x || !y || !z || !origin || !keys_out)) || !(size > 0))
+		return TDV_ERR_ARG;
+	const double fac = (double)(1ull << 63) / size;
+	for (size_t i = 0; i < n; i++)
+		peano_key_words(x[i], y[i], z[i], origin[0], origin[1], origin[2],
+				fac, reversed, &keys_out[2 * i], &keys_out[2 * i + 1]);
+	return 0;
+}
+
 int tdv_peano_keys(tdv_context *c, size_t n, const double *x, const double *y,
 		const double *z, const double origin[3], double size, int reversed,
 		uint64_t *keys_out)
@@ -1147,6 +1160,19 @@ int tdv_ewald_correction(tdv_context *c, size_t n, const double *dr,
 	return r;
 }
 
+int tdv_ewald_correction_host(const tdv_context *c, size_t n,
+		const double *dr, double *f)
+{
+	if (!c || !dr || !f || !c->have_ewald
+			|| c->ewald_host.size() != 4 * (size_t)TDV_NE3)
+		return TDV_ERR_ARG;
+	const double b2e = 2 * TDV_NE / c->par.boxsize;
+	for (size_t i = 0; i < n; i++)
+		ewald_corr(c->ewald_host.data(), dr[3 * i], dr[3 * i + 1],
+				dr[3 * i + 2], b2e, f + 3 * i);
+	return 0;
+}
+
 int tdv_get_ewald_table(tdv_context *c, double *t4)
 {
 	if (!c || !t4 || !c->have_ewald)
@@ -1182,6 +1208,7 @@ int tdv_set_ewald_table(tdv_context *c, const double *t4)
 	TDV_HIP(c, hipMemcpyAsync(c->ewald, h.data(), h.size() * 8,
 			hipMemcpyHostToDevice, c->stream));
 	TDV_HIP(c, hipStreamSynchronize(c->stream));
+	c->ewald_host.swap(h);
 	c->have_ewald = 1;
 	c->ewald16_ok = 0;               /* the FP32 records follow the table */
 	return 0;

@@ -134,6 +134,8 @@ struct tdv_context {
 	int top32_cap;
 
 	double *ewald;                   /* 4 * TDV_NE3, normalised */
+	std::vector<double> ewald_host;  /* the same on the host (per-element
+	                                    Ewald_Correction) */
 	int have_ewald;
 
 	/* FP32 group walk: fixed-point records */

@@ -36,45 +36,8 @@ __global__ void __launch_bounds__(256) k_peano_keys(size_t n,
 	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
-	uint64_t X0 = (uint64_t)((x[i] - o0) * fac);
-	uint64_t X1 = (uint64_t)((y[i] - o1) * fac);
-	uint64_t X2 = (uint64_t)((z[i] - o2) * fac);
-	hilbert_transpose(X0, X1, X2);
-	uint64_t lo = 0, hi = 0;
-	if (!reversed) {
-		/* column of bit b (21..62) -> key bits [3b-61, 3b-59], X0 highest */
-#pragma unroll 6
-		for (int b = 62; b >= 21; b--) {
-			const uint64_t col = ((X0 >> b) & 1) << 2 | ((X1 >> b) & 1) << 1
-					| ((X2 >> b) & 1);
-			const int p = 3 * b - 61;
-			if (p >= 64)
-				hi |= col << (p - 64);
-			else if (p <= 61)
-				lo |= col << p;
-			else {
-				lo |= col << p;
-				hi |= col >> (64 - p);
-			}
-		}
-	} else {
-		/* level k = 63-b at bits [3k, 3k+2]; level 42 loses its top bit */
-#pragma unroll 6
-		for (int b = 21; b <= 62; b++) {
-			const uint64_t col = ((X0 >> b) & 1) << 2 | ((X1 >> b) & 1) << 1
-					| ((X2 >> b) & 1);
-			const int p = 3 * (63 - b);
-			if (p >= 64) {
-				if (p - 64 <= 63)
-					hi |= col << (p - 64);
-			} else if (p <= 61) {
-				lo |= col << p;
-			} else {
-				lo |= col << p;
-				hi |= col >> (64 - p);
-			}
-		}
-	}
+	uint64_t lo, hi;
+	peano_key_words(x[i], y[i], z[i], o0, o1, o2, fac, reversed, &lo, &hi);
 	out[2 * i] = lo;
 	out[2 * i + 1] = hi;
 }

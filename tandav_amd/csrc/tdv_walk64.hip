@@ -249,49 +249,6 @@ __global__ void __launch_bounds__(256) k_walk64(int nact,
 	count_interactions(a.cost, counter);
 }
 
-/* Ewald_Correction, Gravity/periodic.c:35-113 (table already / Box^2) */
-__device__ __forceinline__ void ewald_corr(const double *__restrict__ tab,
-		double dx, double dy, double dz, double box2ewald, double f[3])
-{
-	double sg[3] = { -1, -1, -1 };
-	if (dx < 0) {
-		dx = -dx;
-		sg[0] = 1;
-	}
-	if (dy < 0) {
-		dy = -dy;
-		sg[1] = 1;
-	}
-	if (dz < 0) {
-		dz = -dz;
-		sg[2] = 1;
-	}
-	double u = dx * box2ewald, v = dy * box2ewald, q = dz * box2ewald;
-	int i = (int)u, j = (int)v, k = (int)q;
-	if (i >= TDV_NE)
-		i = TDV_NE - 1;
-	if (j >= TDV_NE)
-		j = TDV_NE - 1;
-	if (k >= TDV_NE)
-		k = TDV_NE - 1;
-	u -= i;
-	v -= j;
-	q -= k;
-	const double w0 = (1 - u) * (1 - v) * (1 - q), w1 = (1 - u) * (1 - v) * q,
-			w2 = (1 - u) * v * (1 - q), w3 = (1 - u) * v * q,
-			w4 = u * (1 - v) * (1 - q), w5 = u * (1 - v) * q,
-			w6 = u * v * (1 - q), w7 = u * v * q;
-	const int b = (i * TDV_NE1 + j) * TDV_NE1 + k;
-	const int di = TDV_NE1 * TDV_NE1, dj = TDV_NE1;
-#pragma unroll
-	for (int d = 0; d < 3; d++) {
-		const double *F = tab + d * TDV_NE3 + b;
-		f[d] = (w0 * F[0] + w1 * F[1] + w2 * F[dj] + w3 * F[dj + 1]
-				+ w4 * F[di] + w5 * F[di + 1] + w6 * F[di + dj]
-				+ w7 * F[di + dj + 1]) * sg[d];
-	}
-}
-
 __device__ __forceinline__ void ewald_cube(Acc &a, const double *tab,
 		double dx, double dy, double dz, double m, const WalkParams &w)
 {

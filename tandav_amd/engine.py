@@ -137,6 +137,9 @@ def load_library(path=LIB_PATH):
     L.tdv_sort_keys.argtypes = [_ctx, _sz, _vp, _vp]
     L.tdv_sort_keys_noalloc.argtypes = [_sz, _vp, _vp, _vp]
     L.tdv_ewald_correction.argtypes = [_ctx, _sz, _dp, _dp]
+    L.tdv_peano_keys_host.argtypes = [_sz, _dp, _dp, _dp, _dp,
+                                      ctypes.c_double, ctypes.c_int, _vp]
+    L.tdv_ewald_correction_host.argtypes = [_ctx, _sz, _dp, _dp]
     L.tdv_get_ewald_table.argtypes = [_ctx, _dp]
     L.tdv_set_ewald_table.argtypes = [_ctx, _dp]
     L.tdv_get_domain.argtypes = [_ctx, _dp, _dp, _dp]
@@ -193,6 +196,22 @@ def load_library(path=LIB_PATH):
 
 def _d(a):
     return a.ctypes.data_as(_dp)
+
+
+def peano_keys_host(x, y, z, origin, size, reversed_=False, lib=None):
+    """Peano_Key / Reversed_Peano_Key (peano.c:122-276) computed on the host
+    by the library's own transform (tdv_peano_keys_host): no device, no
+    context.  Returns (n, 2) uint64 words (low, high)."""
+    L = lib or load_library()
+    x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
+    o = np.ascontiguousarray(origin, dtype=np.float64)
+    out = np.zeros((x.size, 2), np.uint64)
+    rc = L.tdv_peano_keys_host(x.size, _d(x), _d(y), _d(z), _d(o),
+                               float(size), int(reversed_),
+                               out.ctypes.data_as(ctypes.c_void_p))
+    if rc != 0:
+        raise TDVError("tdv_peano_keys_host: error %d" % rc)
+    return out
 
 
 class SnapshotHeader(ctypes.Structure):
@@ -549,6 +568,10 @@ class Engine:
                  "tdv_peano_keys")
         return out
 
+    def peano_keys_host(self, x, y, z, origin, size, reversed_=False):
+        """The same keys computed on the host (tdv_peano_keys_host)."""
+        return peano_keys_host(x, y, z, origin, size, reversed_, self.lib)
+
     def sort_keys(self, keys2):
         """Qsort_Index(perm, keys, n, 16, cmp_peanoKeys) (sort.c:90-116)."""
         k = np.ascontiguousarray(keys2, dtype=np.uint64)
@@ -565,6 +588,17 @@ class Engine:
         f = np.zeros_like(dr)
         self._ok(self.lib.tdv_ewald_correction(self._c, dr.shape[0], _d(dr),
                                                _d(f)), "tdv_ewald_correction")
+        return f
+
+    def ewald_correction_host(self, dr):
+        """The same lookup on the host (tdv_ewald_correction_host)."""
+        dr = np.ascontiguousarray(dr, dtype=np.float64).reshape(-1, 3)
+        f = np.zeros_like(dr)
+        rc = self.lib.tdv_ewald_correction_host(self._c, dr.shape[0], _d(dr),
+                                                _d(f))
+        if rc != 0:
+            raise TDVError("tdv_ewald_correction_host: error %d (no table)"
+                           % rc)
         return f
 
     def ewald_table(self):

@@ -17,8 +17,10 @@
  * device work inside "omp single", whose implicit barrier every thread
  * passes; the threadprivate Sig (signal.h:23) is read on the thread that
  * runs the single and reset on every thread where the reference resets it.
- * The per-element functions (Peano_Key, Ewald_Correction, ...) may be called
- * concurrently and serialise on one named critical section.
+ * The per-element functions (Peano_Key, Reversed_Peano_Key,
+ * Ewald_Correction, ...) compute on the host with the engine's own code
+ * (tdv_peano_keys_host, tdv_ewald_correction_host): no device round trip,
+ * any thread at once.
  *
  * Errors.  A failing engine call reports the engine's message through
  * Assert (macro.h:15, Assert_Info aux.c:49-77), the reference's own abort
@@ -198,13 +200,14 @@ static peanoKey one_key(const Float px, const Float py, const Float pz,
 		const int reversed)
 {
 	uint64_t w[2] = { 0, 0 };
-	#pragma omp critical (tandav_engine)
-	TDV_CHECK(tdv_peano_keys(context(), 1, &px, &py, &pz, Domain.Origin,
-			Domain.Size, reversed, w));
+	const int rc = tdv_peano_keys_host(1, &px, &py, &pz, Domain.Origin,
+			Domain.Size, reversed, w);
+	Assert(rc == TDV_OK, "tandav engine: tdv_peano_keys_host failed (%d): "
+			"Domain.Size %g", rc, Domain.Size);
 	return ((peanoKey)w[1] << 64) | w[0];
 }
 
-/* Peano_Key (peano.c:122-195), one key through the batched device call */
+/* Peano_Key (peano.c:122-195), on the host with the engine's transform */
 peanoKey Peano_Key(const Float px, const Float py, const Float pz)
 {
 	return one_key(px, py, pz, 0);
@@ -564,12 +567,12 @@ void Gravity_Periodic_Init(void)
 	free(tab);
 }
 
-/* Ewald_Correction (Gravity/periodic.c:35-113), one separation through the
- * batched device call. */
+/* Ewald_Correction (Gravity/periodic.c:35-113), one separation looked up
+ * on the host in the engine's copy of the table (set by
+ * Gravity_Periodic_Init). */
 void Ewald_Correction(const Float dr[3], Float f[3])
 {
-	#pragma omp critical (tandav_engine)
-	TDV_CHECK(tdv_ewald_correction(context(), 1, dr, f));
+	TDV_CHECK(tdv_ewald_correction_host(context(), 1, dr, f));
 }
 #endif // PERIODIC
 

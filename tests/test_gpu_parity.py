@@ -81,6 +81,21 @@ def test_ewald_table_and_lookup(ewald):
                    boxsize=2.0, ewald_table=ewald, nthreads=1)
     want = np.array([s.ewald_correction(d) for d in dr[:2000]])
     np.testing.assert_array_equal(got[:2000], want)
+    # the per-element form on the host: the same code, the same table
+    np.testing.assert_array_equal(e.ewald_correction_host(dr), got)
+
+
+def test_host_keys_match_device_keys():
+    """tdv_peano_keys_host (the binding's per-element Peano_Key) computes on
+    the host what the device computes, bit for bit."""
+    g = np.random.Generator(np.random.Philox(24))
+    x = g.uniform(-3.0, 5.0, (100000, 3))
+    e = _engine()
+    for rev in (False, True):
+        np.testing.assert_array_equal(
+            e.peano_keys_host(x[:, 0], x[:, 1], x[:, 2], [-3.0] * 3, 8.0001,
+                              rev),
+            e.peano_keys(x[:, 0], x[:, 1], x[:, 2], [-3.0] * 3, 8.0001, rev))
 
 
 def _pipeline_pair(pos, mass, periodic, ewald, prepare=True, **kw):
