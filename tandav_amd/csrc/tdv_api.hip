@@ -395,6 +395,7 @@ void tdv_destroy(tdv_context *c)
 	dfree(c->d_nactive);
 	dfree(c->d_queue);
 	dfree(c->partial);
+	dfree(c->wmask);
 	dfree(c->gkey);
 	dfree(c->gorder);
 	dfree(c->long_count);

@@ -83,6 +83,8 @@ struct tdv_context {
 	                                    walk and captured graphs read it) */
 	unsigned *d_queue;               /* work-queue counter of the walk */
 	double4 *partial;                /* split walk: TDV_SPLIT_ITEMS x 64 */
+	unsigned long long *wmask;       /* split walk: wanted top nodes,
+	                                    TDV_WANT_WORDS per group */
 	uint64_t *gkey;                  /* FP32 walk: 2 per group, cost keys */
 	uint32_t *gorder;                /* groups by descending expected cost */
 	size_t gcap;                     /* groups gkey and gorder hold */
@@ -217,6 +219,8 @@ void tdv_graph_drop(tdv_context *c);
                                             one wavefront each */
 #define TDV_LONG_GRID 2048               /* waves of that kernel */
 #define TDV_SPLIT_AUTO_MAX 64            /* parts per group, split 0 */
+#define TDV_WANT_WORDS 4                 /* a split walk's wanted-top-node
+                                            mask: up to 256 top nodes */
 #define TDV_ORDER_MIN_GROUPS 2048        /* FP32 walks of this many whole
                                             groups ... */
 #define TDV_ORDER_MAX_GROUPS (1 << 20)   /* ... to this many start the

@@ -804,6 +804,24 @@ __device__ __forceinline__ void node32(Acc32 &a, float m, float4 qa,
 	a.z = fmaf(-ri4, tuz, fmaf(g, uz, a.z));
 }
 
+/* Whether a lane's particle wants top node j walked (its subtree) rather
+ * than taken whole, and the separation; the walk and k_walk32_want
+ * evaluate it alike. */
+__device__ __forceinline__ bool top_want(const int4 tc, float ts,
+		const int4 me, float crit, const Walk32Params &w, float &rx,
+		float &ry, float &rz)
+{
+	const float tm = __int_as_float(tc.w);
+	rx = fdiff(tc.x, me.x);
+	ry = fdiff(tc.y, me.y);
+	rz = fdiff(tc.z, me.z);
+	const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+	const bool want = w.use_bh ? (ts * ts > r2 * w.theta_bh)
+			: rel_open(tm, ts, r2, crit, w);
+	/* the top node holds the particle (or is close): walk it */
+	return want || r2 < 3.f * ts * ts;
+}
+
 /* One wave walks `group` (64 active particles, one per lane) over part
  * `part` of `nparts`: every subtree the group walks is cut into nparts
  * contiguous node ranges and this wave takes range `part` of each, so
@@ -825,7 +843,8 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 		double *__restrict__ gy, double *__restrict__ gz,
 		double *__restrict__ cost, unsigned long long *counter,
 		const int4 *__restrict__ top32, const float4 *__restrict__ tq32,
-		const float4 *__restrict__ p32o)
+		const float4 *__restrict__ p32o,
+		const unsigned long long *__restrict__ wmask)
 {
 	if (*T.nnodes < 0)                   /* the build overflowed: no walk */
 		return;
@@ -841,19 +860,30 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 	const int every = w.carry_every;
 	const float inv_parts = 1.f / (float)nparts;
 
+	/* a split walk's parts other than 0 walk only the top nodes the group
+	 * wants walked, which k_walk32_want found once for the group */
+	const unsigned long long *wm = part != 0 && wmask
+			&& ntop <= 64 * TDV_WANT_WORDS
+			? wmask + (size_t)group * TDV_WANT_WORDS : nullptr;
 	for (int j = 0; j < ntop; j++) {
+		if (wm) {
+			const unsigned long long bits = wm[j >> 6] >> (j & 63);
+			if (!bits) {
+				j |= 63;                 /* the next word */
+				continue;
+			}
+			j += (int)__builtin_ctzll(bits);
+			if (j >= ntop)
+				break;
+			if (T.npart[j] <= w.vs)      /* part 0's */
+				continue;
+		} else {
 		const int4 tc = top32[j];
 		const float tm = __int_as_float(tc.w);
 		/* dom_f / 2^level, exact: one ldexp instead of a division */
 		const float ts = ldexpf(w.dom_f, -T.level[j]);
-		const float rx = fdiff(tc.x, me.x), ry = fdiff(tc.y, me.y),
-				rz = fdiff(tc.z, me.z);
-		const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
-		bool want = w.use_bh ? (ts * ts > r2 * w.theta_bh)
-				: rel_open(tm, ts, r2, crit, w);
-		/* the top node holds the particle (or is close): walk it */
-		want = want || r2 < 3.f * ts * ts;
-		if (!__any(want)) {
+		float rx, ry, rz;
+		if (!__any(top_want(tc, ts, me, crit, w, rx, ry, rz))) {
 			if (part != 0)
 				continue;
 			newton32(a, tm, rx, ry, rz, w);
@@ -884,6 +914,7 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 				tick(a, every);
 			}
 			continue;
+		}
 		}
 		const int start = T.target[j];
 		int lo = start, hi = T.term[j];
@@ -1139,10 +1170,12 @@ __device__ __forceinline__ void walk32_body(int group, int part, int nparts,
 		double *__restrict__ cost, unsigned long long *counter, \
 		const int4 *__restrict__ top32, const float4 *__restrict__ tq32, \
 		const float4 *__restrict__ p32o, const uint32_t *__restrict__ gorder
-#define TDV_WALK32_CALL_PART(g, part, nparts, partial, nslot) \
+#define TDV_WALK32_CALL_PARTW(g, part, nparts, partial, nslot, wmask) \
 		walk32_body(g, part, nparts, partial, nslot, nact, active, p32, \
 		last_acc, T, rec, tab, w, ax, ay, az, gx, gy, gz, cost, \
-		counter, top32, tq32, p32o)
+		counter, top32, tq32, p32o, wmask)
+#define TDV_WALK32_CALL_PART(g, part, nparts, partial, nslot) \
+		TDV_WALK32_CALL_PARTW(g, part, nparts, partial, nslot, nullptr)
 #define TDV_WALK32_CALL(g) TDV_WALK32_CALL_PART(g, 0, 1, nullptr, 0)
 /* the i-th group to walk: by descending expected cost when ordered */
 #define TDV_GROUP(i) (gorder ? (int)gorder[i] : (i))
@@ -1176,6 +1209,45 @@ __host__ __device__ __forceinline__ int walk32_parts(int ngroups, int split)
 	return k < 1 ? 1 : k < kmax ? k : kmax;
 }
 
+/* For a split walk: each group's wanted top nodes (bit j of word j / 64:
+ * some lane wants top node j's subtree walked), evaluated once per group
+ * instead of once per part; the parts other than 0 then go straight to
+ * those top nodes.  At most 64 TDV_WANT_WORDS top nodes (more: every part
+ * evaluates them itself). */
+__global__ void __launch_bounds__(64) k_walk32_want(
+		const int *__restrict__ nact_dev, int split,
+		const int *__restrict__ active, const int4 *__restrict__ p32,
+		const double *__restrict__ last_acc, TopNodes T,
+		const int4 *__restrict__ top32, Walk32Params w,
+		unsigned long long *__restrict__ wmask)
+{
+	const int group = blockIdx.x;
+	const int nact = *nact_dev;
+	const int ngroups = (nact + 63) / 64;
+	if (group >= ngroups || walk32_parts(ngroups, split) == 1
+			|| *T.nnodes < 0)
+		return;
+	const int ntop = *T.ntop;
+	if (ntop > 64 * TDV_WANT_WORDS)
+		return;
+	const int gid = group * 64 + (int)threadIdx.x;
+	const int ip = gid < nact ? active[gid] : active[nact - 1];
+	const int4 me = p32[ip];
+	const float crit = (float)(last_acc[ip] * (double)w.alpha_rel);
+	for (int q = 0; q < TDV_WANT_WORDS && q * 64 < ntop; q++) {
+		unsigned long long bits = 0;
+		for (int b = 0; b < 64 && q * 64 + b < ntop; b++) {
+			const int j = q * 64 + b;
+			const float ts = ldexpf(w.dom_f, -T.level[j]);
+			float rx, ry, rz;
+			if (__any(top_want(top32[j], ts, me, crit, w, rx, ry, rz)))
+				bits |= 1ull << b;
+		}
+		if (threadIdx.x == 0)
+			wmask[(size_t)group * TDV_WANT_WORDS + q] = bits;
+	}
+}
+
 /* Persistent form: a fixed grid whose waves take work items (a 64-particle
  * group, or one part of one, see walk32_body) from a work queue (one atomic
  * counter) until it is exhausted, with the active count read from device
@@ -1184,7 +1256,8 @@ __host__ __device__ __forceinline__ int walk32_parts(int ngroups, int split)
  * its parts run side by side.  Every wave leaves once the counter has
  * passed the last item. */
 __global__ void __launch_bounds__(64) k_walk32_queue(const int *__restrict__ nact_dev, unsigned *__restrict__ queue,
-		int split, double4 *__restrict__ partial, TDV_WALK32_ARGS)
+		int split, double4 *__restrict__ partial,
+		const unsigned long long *__restrict__ wmask, TDV_WALK32_ARGS)
 {
 	nact = *nact_dev;
 	const int ngroups = (nact + 63) / 64;
@@ -1203,8 +1276,8 @@ __global__ void __launch_bounds__(64) k_walk32_queue(const int *__restrict__ nac
 		if (item >= nitems)
 			break;
 		const int gi = item / nparts;
-		TDV_WALK32_CALL_PART(TDV_GROUP(gi), item - gi * nparts, nparts,
-				partial, nslot);
+		TDV_WALK32_CALL_PARTW(TDV_GROUP(gi), item - gi * nparts, nparts,
+				partial, nslot, wmask);
 	}
 }
 
@@ -1221,26 +1294,29 @@ __global__ void __launch_bounds__(64) k_walk32_queue(const int *__restrict__ nac
 	if (item >= ngroups * nparts)                                          \
 		return;                                                            \
 	const int gi = item / nparts;                                          \
-	TDV_WALK32_CALL_PART(TDV_GROUP(gi), item - gi * nparts, nparts, partial, \
-			ngroups * 64)
+	TDV_WALK32_CALL_PARTW(TDV_GROUP(gi), item - gi * nparts, nparts,       \
+			partial, ngroups * 64, wmask)
 
 __global__ void __launch_bounds__(64) k_walk32_items(
 		const int *__restrict__ nact_dev, int split,
-		double4 *__restrict__ partial, TDV_WALK32_ARGS)
+		double4 *__restrict__ partial,
+		const unsigned long long *__restrict__ wmask, TDV_WALK32_ARGS)
 {
 	TDV_WALK32_ITEMS_BODY;
 }
 
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
 k_walk32_items_w6(const int *__restrict__ nact_dev, int split,
-		double4 *__restrict__ partial, TDV_WALK32_ARGS)
+		double4 *__restrict__ partial,
+		const unsigned long long *__restrict__ wmask, TDV_WALK32_ARGS)
 {
 	TDV_WALK32_ITEMS_BODY;
 }
 
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8)))
 k_walk32_items_w8(const int *__restrict__ nact_dev, int split,
-		double4 *__restrict__ partial, TDV_WALK32_ARGS)
+		double4 *__restrict__ partial,
+		const unsigned long long *__restrict__ wmask, TDV_WALK32_ARGS)
 {
 	TDV_WALK32_ITEMS_BODY;
 }
@@ -1420,9 +1496,14 @@ int tdv_walk32(tdv_context *c, int use_bh)
 		TDV_HIP(c, hipMalloc((void **)&c->gorder,
 				c->gcap * sizeof(uint32_t)));
 	}
-	if (!c->partial)                     /* split walks' partial sums */
+	if (!c->partial) {                   /* split walks' partial sums */
 		TDV_HIP(c, hipMalloc((void **)&c->partial,
 				(size_t)TDV_SPLIT_ITEMS * 64 * sizeof(double4)));
+		/* and their groups' wanted top nodes: a walk splits with at most
+		 * TDV_SPLIT_ITEMS / 2 groups (two parts each) */
+		TDV_HIP(c, hipMalloc((void **)&c->wmask, (size_t)TDV_SPLIT_ITEMS / 2
+				* TDV_WANT_WORDS * sizeof(unsigned long long)));
+	}
 	if (c->n32_cap < c->node_cap) {
 		tdv_graph_drop(c);               /* the records move */
 		if (c->a32)
@@ -1587,6 +1668,16 @@ int tdv_walk32(tdv_context *c, int use_bh)
 				c->gorder));
 		gord = c->gorder;
 	}
+	if (c->capturing || parts) {
+		/* the groups' wanted top nodes (a captured walk finds them on
+		 * every replay; the kernel leaves at once when the active groups
+		 * do not split) */
+		hipLaunchKernelGGL(k_walk32_want, dim3(c->capturing
+				? (unsigned)TDV_SPLIT_ITEMS / 2 : (unsigned)ngroups_h),
+				dim3(64), 0, c->stream, c->d_nactive, split, c->active,
+				c->p32, c->last_acc, c->top, c->top32, w, c->wmask);
+		TDV_TRY(tdv_check(c, hipGetLastError(), "k_walk32_want"));
+	}
 	if (!c->par.fp32_queue && (c->capturing || parts)) {
 		/* one workgroup per item; captured: enough for any active set */
 		const size_t items = c->capturing
@@ -1597,7 +1688,8 @@ int tdv_walk32(tdv_context *c, int use_bh)
 		hipLaunchKernelGGL(k_walk32_queue_reset, dim3(1), dim3(64), 0,
 				c->stream, c->d_queue, c->d_count);
 		hipLaunchKernelGGL(kern, dim3((unsigned)items), dim3(64), 0,
-				c->stream, c->d_nactive, split, c->partial, 0, c->active,
+				c->stream, c->d_nactive, split, c->partial, c->wmask, 0,
+				c->active,
 				c->p32, c->last_acc, c->top,
 				reinterpret_cast<const Node32 *>(c->rec32), c->ewald16, w, c->acc[0], c->acc[1], c->acc[2], c->gacc[0],
 				c->gacc[1], c->gacc[2], c->cost, c->d_count, c->top32,
@@ -1614,7 +1706,8 @@ int tdv_walk32(tdv_context *c, int use_bh)
 		hipLaunchKernelGGL(k_walk32_queue_reset, dim3(1), dim3(64), 0,
 				c->stream, c->d_queue, c->d_count);
 		hipLaunchKernelGGL(k_walk32_queue, dim3(TDV_QUEUE_GRID), dim3(64), 0,
-				c->stream, c->d_nactive, c->d_queue, split, c->partial, 0,
+				c->stream, c->d_nactive, c->d_queue, split, c->partial,
+				c->wmask, 0,
 				c->active, c->p32, c->last_acc, c->top,
 				reinterpret_cast<const Node32 *>(c->rec32), c->ewald16, w, c->acc[0], c->acc[1], c->acc[2],
 				c->gacc[0], c->gacc[1], c->gacc[2], c->cost, c->d_count,
