@@ -52,12 +52,17 @@ __device__ __forceinline__ unsigned key_triplet(Key2 k, int L)
 }
 
 /* Skilling (2004) inverse-undo + Gray encode on the three transpose words,
- * the transform of Peano_Key (peano.c:131-170). */
+ * the transform of Peano_Key (peano.c:131-170), for the bits the key keeps.
+ * The reference runs the undo down to bit 1; step s changes only bits below
+ * s, so the steps below 22 touch only bits 0-20, which the 42-level key
+ * (bits 21-62 of each word) drops, and the Gray encode combines bits only
+ * downwards: stopping at s = 22 gives the same key in two thirds of the
+ * steps. */
 __host__ __device__ inline void hilbert_transpose(uint64_t &X0, uint64_t &X1,
 		uint64_t &X2)
 {
-#pragma unroll 4
-	for (int s = 63; s >= 1; s--) {
+#pragma unroll 6
+	for (int s = 63; s >= 22; s--) {
 		const uint64_t q = 1ull << s, low = q - 1;
 		if (X0 & q)
 			X0 ^= low;

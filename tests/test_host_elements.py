@@ -32,6 +32,27 @@ def test_host_keys_match_oracle():
             oracle.peano_keys(p[:, 0], p[:, 1], p[:, 2], [0, 0, 0], 1.0, rev))
 
 
+def test_host_keys_on_dyadic_boundaries():
+    """Points on and next to the cell walls of levels 1-42 (the bits the
+    transform's undo steps compare), where a step left out or one too many
+    would change the key."""
+    g = np.random.Generator(np.random.Philox(32))
+    lv = g.integers(1, 43, 60000)
+    k = np.floor(g.random(60000) * 2.0 ** np.minimum(lv, 52)).astype(np.float64)
+    wall = k / 2.0 ** lv
+    x = np.stack([wall, g.random(60000), g.random(60000)], 1)
+    for c in range(3):
+        xx = np.roll(x, c, axis=1)
+        for nudge in (0.0, 1e-15, -1e-15):
+            p = np.clip(xx + nudge, 0.0, 0.999999999999)
+            for rev in (False, True):
+                np.testing.assert_array_equal(
+                    engine.peano_keys_host(p[:, 0], p[:, 1], p[:, 2],
+                                           [0, 0, 0], 1.0, rev),
+                    oracle.peano_keys(p[:, 0], p[:, 1], p[:, 2], [0, 0, 0],
+                                      1.0, rev))
+
+
 def test_host_keys_reject_bad_size():
     with pytest.raises(engine.TDVError):
         engine.peano_keys_host([0.5], [0.5], [0.5], [0, 0, 0], 0.0)
