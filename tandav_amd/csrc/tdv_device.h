@@ -61,8 +61,8 @@ __device__ __forceinline__ unsigned key_triplet(Key2 k, int L)
 __host__ __device__ inline void hilbert_transpose(uint64_t &X0, uint64_t &X1,
 		uint64_t &X2)
 {
-#pragma unroll 6
-	for (int s = 63; s >= 22; s--) {
+#pragma unroll
+	for (int s = 63; s >= 22; s--) {      /* unrolled: constant masks */
 		const uint64_t q = 1ull << s, low = q - 1;
 		if (X0 & q)
 			X0 ^= low;
