@@ -162,21 +162,25 @@ __device__ __forceinline__ int dprev_of(const uint64_t *key, int i, int first,
 	return div_level(load_key(key, i - 1), load_key(key, i));
 }
 
-/* particles in the cell of level lvl that ends at i-1 (bounded scan:
- * returns vs+1 when the cell holds more than vs particles) */
+/* particles in the cell of level lvl that ends at i-1, counted back no
+ * further than first (bounded: vs+1 when the cell holds more than vs
+ * particles).  The keys are sorted, so the cell's keys are the run ending
+ * at i-1 of keys that share key[i-1]'s first lvl levels: a binary search
+ * over the last vs+1 keys finds its start in 7 dependent loads, where a
+ * scan back took up to 64. */
 __device__ __forceinline__ int cell_count_back(const uint64_t *key, int i,
 		int first, int lvl, int vs)
 {
-	int s = i - 1;
-	Key2 ks = load_key(key, s);
-	while (s > first && i - s <= vs) {
-		const Key2 kp = load_key(key, s - 1);
-		if (div_level(kp, ks) <= lvl)
-			break;
-		ks = kp;
-		s--;
+	const Key2 kl = load_key(key, i - 1);
+	int lo = i - vs - 1 > first ? i - vs - 1 : first, hi = i - 1;
+	while (lo < hi) {                    /* key[hi] is in the cell */
+		const int mid = (lo + hi) >> 1;
+		if (div_level(load_key(key, mid), kl) > lvl)
+			hi = mid;
+		else
+			lo = mid + 1;
 	}
-	return (i - s <= vs) ? i - s : vs + 1;
+	return i - hi <= vs ? i - hi : vs + 1;
 }
 
 __global__ void __launch_bounds__(256) k_tree_marks(size_t n,
