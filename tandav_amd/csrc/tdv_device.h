@@ -53,16 +53,18 @@ __device__ __forceinline__ unsigned key_triplet(Key2 k, int L)
 
 /* Skilling (2004) inverse-undo + Gray encode on the three transpose words,
  * the transform of Peano_Key (peano.c:131-170), for the bits the key keeps.
- * The reference runs the undo down to bit 1; step s changes only bits below
- * s, so the steps below 22 touch only bits 0-20, which the 42-level key
- * (bits 21-62 of each word) drops, and the Gray encode combines bits only
- * downwards: stopping at s = 22 gives the same key in two thirds of the
- * steps. */
+ * The reference runs the undo down to bit 1 on the 64-bit words; step s
+ * changes only bits below s and the Gray encode combines bits only
+ * downwards, so bits 0-20, which the 42-level key (bits 21-62 of each word)
+ * drops, never reach the others.  The words come in shifted right by 21
+ * (their bits 21-63 as bits 0-42) and the undo runs steps 42..1 on them:
+ * the same key in two thirds of the steps, 31 of them on the low 32 bits
+ * only. */
 __host__ __device__ inline void hilbert_transpose(uint64_t &X0, uint64_t &X1,
 		uint64_t &X2)
 {
 #pragma unroll
-	for (int s = 63; s >= 22; s--) {      /* unrolled: constant masks */
+	for (int s = 42; s >= 1; s--) {       /* unrolled: constant masks */
 		const uint64_t q = 1ull << s, low = q - 1;
 		if (X0 & q)
 			X0 ^= low;
@@ -96,53 +98,52 @@ __host__ __device__ inline void hilbert_transpose(uint64_t &X0, uint64_t &X1,
 	X0 ^= t;
 }
 
+/* bit j of the low 21 bits of x to bit 3j (the usual shift-and-mask
+ * dilation of Morton codes: five steps instead of 21 single-bit moves) */
+__host__ __device__ inline uint64_t dilate3_21(uint64_t x)
+{
+	x &= 0x1fffffull;
+	x = (x | x << 32) & 0x1f00000000ffffull;
+	x = (x | x << 16) & 0x1f0000ff0000ffull;
+	x = (x | x << 8) & 0x100f00f00f00f00full;
+	x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+	x = (x | x << 2) & 0x1249249249249249ull;
+	return x;
+}
+
 /* The standard or reversed 128-bit key of one position (peano.c:122-276):
- * the same code in k_peano_keys and on the host (tdv_peano_keys_host). */
+ * the same code in k_peano_keys and on the host (tdv_peano_keys_host).
+ * After the transform, the key is the three coordinates' bits 62..21
+ * interleaved, X0 the most significant of each level's triplet.  With
+ * u_c = X_c >> 21 (its low 42 bits), interleaving the low and the high
+ * 21 bits of the three u_c gives 63-bit words L and H, and the standard
+ * key (levels from the top, at bits 127..2) is (H 2^63 + L) 4.  The
+ * reversed key (level k at bits 3k..3k+2, level 42 losing its top bit) is
+ * the same interleave of the bit-reversed u_c, times 8, modulo 2^128. */
 __host__ __device__ inline void peano_key_words(double x, double y, double z,
 		double o0, double o1, double o2, double fac, int reversed,
 		uint64_t *lo_out, uint64_t *hi_out)
 {
-	uint64_t X0 = (uint64_t)((x - o0) * fac);
-	uint64_t X1 = (uint64_t)((y - o1) * fac);
-	uint64_t X2 = (uint64_t)((z - o2) * fac);
-	hilbert_transpose(X0, X1, X2);
-	uint64_t lo = 0, hi = 0;
-	if (!reversed) {
-		/* column of bit b (21..62) -> key bits [3b-61, 3b-59], X0 highest */
-#pragma unroll 6
-		for (int b = 62; b >= 21; b--) {
-			const uint64_t col = ((X0 >> b) & 1) << 2 | ((X1 >> b) & 1) << 1
-					| ((X2 >> b) & 1);
-			const int p = 3 * b - 61;
-			if (p >= 64)
-				hi |= col << (p - 64);
-			else if (p <= 61)
-				lo |= col << p;
-			else {
-				lo |= col << p;
-				hi |= col >> (64 - p);
-			}
-		}
-	} else {
-		/* level k = 63-b at bits [3k, 3k+2]; level 42 loses its top bit */
-#pragma unroll 6
-		for (int b = 21; b <= 62; b++) {
-			const uint64_t col = ((X0 >> b) & 1) << 2 | ((X1 >> b) & 1) << 1
-					| ((X2 >> b) & 1);
-			const int p = 3 * (63 - b);
-			if (p >= 64) {
-				if (p - 64 <= 63)
-					hi |= col << (p - 64);
-			} else if (p <= 61) {
-				lo |= col << p;
-			} else {
-				lo |= col << p;
-				hi |= col >> (64 - p);
-			}
-		}
+	uint64_t u0 = (uint64_t)((x - o0) * fac) >> 21;
+	uint64_t u1 = (uint64_t)((y - o1) * fac) >> 21;
+	uint64_t u2 = (uint64_t)((z - o2) * fac) >> 21;
+	hilbert_transpose(u0, u1, u2);
+	if (reversed) {                      /* bit j -> bit 41 - j */
+		u0 = __builtin_bitreverse64(u0) >> 22;
+		u1 = __builtin_bitreverse64(u1) >> 22;
+		u2 = __builtin_bitreverse64(u2) >> 22;
 	}
-	*lo_out = lo;
-	*hi_out = hi;
+	const uint64_t L = dilate3_21(u0) << 2 | dilate3_21(u1) << 1
+			| dilate3_21(u2);
+	const uint64_t H = dilate3_21(u0 >> 21) << 2 | dilate3_21(u1 >> 21) << 1
+			| dilate3_21(u2 >> 21);
+	if (!reversed) {
+		*lo_out = L << 2;
+		*hi_out = L >> 62 | H << 1;
+	} else {
+		*lo_out = L << 3;
+		*hi_out = L >> 61 | H << 2;
+	}
 }
 
 /* Ewald_Correction, Gravity/periodic.c:35-113 (table already / Box^2):
