@@ -211,6 +211,18 @@ __device__ __forceinline__ uint64_t rs_load(const uint64_t *k, size_t i,
 	return k[i * stride + off];
 }
 
+/* The tile workgroup b of nb takes.  Workgroups go to the 8 XCDs in turn
+ * (b to XCD b mod 8); numbering the tiles so that each XCD takes one
+ * contiguous range of them keeps neighbouring tiles' histogram and offset
+ * entries (4 bytes apart, digit-major) and the digit runs they write side
+ * by side in one XCD's L2, where they had been partial lines of eight
+ * L2s, each written back or fetched on its own. */
+__device__ __forceinline__ unsigned rs_tile(unsigned b, unsigned nb)
+{
+	const unsigned q = nb / 8, r = nb % 8, x = b % 8, i = b / 8;
+	return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(
 		const uint64_t *__restrict__ keys, int stride, int off, size_t n,
 		int shift, uint32_t *__restrict__ hist, unsigned nblocks)
@@ -218,7 +230,8 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(
 	__shared__ uint32_t h[RS_BINS];
 	h[threadIdx.x] = 0;
 	__syncthreads();
-	const size_t base = (size_t)blockIdx.x * RS_TILE;
+	const unsigned tile = rs_tile(blockIdx.x, nblocks);
+	const size_t base = (size_t)tile * RS_TILE;
 	/* every load issued before the first is used: unconditional loads of
 	 * an index clamped into [0, n) (n >= 1), the count only when i < n */
 	uint64_t kk[RS_ITEMS];
@@ -234,7 +247,7 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(
 			atomicAdd(&h[(unsigned)(kk[it] >> shift) & 255u], 1u);
 	}
 	__syncthreads();
-	hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+	hist[(size_t)threadIdx.x * nblocks + tile] = h[threadIdx.x];
 }
 
 /* One LSD pass's scatter of a tile of RS_TILE keys: every key's rank
@@ -258,10 +271,11 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(
 	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 	for (int q = 0; q < RS_WAVES; q++)
 		wcnt[q][threadIdx.x] = 0;
-	binoff[threadIdx.x] = hscan[(size_t)threadIdx.x * nblocks + blockIdx.x];
+	const unsigned tile = rs_tile(blockIdx.x, nblocks);
+	binoff[threadIdx.x] = hscan[(size_t)threadIdx.x * nblocks + tile];
 	__syncthreads();
 
-	const size_t base = (size_t)blockIdx.x * RS_TILE + (size_t)w * 64 * RS_ITEMS;
+	const size_t base = (size_t)tile * RS_TILE + (size_t)w * 64 * RS_ITEMS;
 	uint64_t kk[RS_ITEMS];
 	uint32_t vv[RS_ITEMS];
 	uint32_t rk[RS_ITEMS];
@@ -338,7 +352,7 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(
 		}
 	}
 	__syncthreads();
-	const size_t tile0 = (size_t)blockIdx.x * RS_TILE;
+	const size_t tile0 = (size_t)tile * RS_TILE;
 	const int nvalid = n - tile0 < (size_t)RS_TILE ? (int)(n - tile0)
 			: RS_TILE;
 	for (int j = threadIdx.x; j < nvalid; j += RS_THREADS) {
