@@ -258,11 +258,13 @@ C4 = dict(n=1 << 23, nhalo=256, seed=11, alpha=0.002,
 def walk32_kernel(waves, queue, split, nactive):
     """Name of the kernel an eager FP32 walk over nactive particles runs:
     the work-queue kernel, the one-workgroup-per-item kernel when the walk
-    splits (parts per group as walk32_parts: up to 16384 items, at most 64
-    parts or `split`), else the plain grid; with the register budget."""
+    splits (parts per group as walk32_parts: up to 32768 items from 4096
+    groups up, 16384 below, at most 64 parts or `split`), else the plain
+    grid; with the register budget."""
     ngroups = (nactive + 63) // 64
     kmax = split if split >= 2 else 64
-    parts = 1 if split == 1 or ngroups <= 0 else max(1, min(16384 // ngroups,
+    budget = 32768 if ngroups >= 4096 else 16384
+    parts = 1 if split == 1 or ngroups <= 0 else max(1, min(budget // ngroups,
                                                              kmax))
     if queue:
         return "k_walk32_queue"

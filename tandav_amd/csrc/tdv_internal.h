@@ -214,7 +214,9 @@ void tdv_graph_drop(tdv_context *c);
                                             waves beyond those resident
                                             start later and find the queue
                                             drained */
-#define TDV_SPLIT_ITEMS 16384            /* work items a split walk aims at */
+#define TDV_SPLIT_ITEMS 32768            /* work items a split walk aims at
+                                            (half of them below ... */
+#define TDV_SPLIT_WIDE_GROUPS 4096       /* ... this many active groups) */
 #define TDV_LONG_CHAIN 256               /* build: longer chain scans go to
                                             one wavefront each */
 #define TDV_LONG_GRID 2048               /* waves of that kernel */
